@@ -1,0 +1,175 @@
+/*
+ * measure_ia_amd.h -- C ABI of the MI355X periodic-box pair-counting engine.
+ *
+ * The reference (MeasureIA, pure Python) has no FFI: its boundary is the
+ * Python method contract of the private box counters that
+ * MeasureIABox.measure_xi_w / measure_xi_multipoles dispatch to
+ * (src/measureia/measure_IA.py:120-160, 218-259).  Every counter takes the
+ * data dict, prepares the arrays (src/measureia/measure_w_box_jk.py:97-155)
+ * and returns or writes DD, S+D, SxD and the jackknife tallies DD_jk,
+ * S+D_jk (measure_w_box_jk.py:157-211).  This header replaces the pair
+ * loop of those counters -- the KDTree neighbour search, the binning and the
+ * jackknife tallies -- and nothing else: input preparation, the analytic
+ * randoms, the estimators and the HDF5 output stay in the host code.
+ *
+ * Conventions
+ *   - Plain C types only.  Functions return 0 on success and a negative
+ *     MIA_E_* code on error; mia_last_error() gives the message.  No C++
+ *     exception crosses this boundary.
+ *   - Arrays marked "device" are HIP device pointers owned by the caller;
+ *     arrays marked "host" are ordinary host memory.  The context owns all
+ *     scratch memory and reuses it between calls.
+ *   - Every launch goes on the caller's stream (NULL = default stream).  The
+ *     count calls read a few sizes back to the host, so they synchronise
+ *     that stream before returning.
+ *   - One context per host thread; a context is not thread-safe.
+ */
+#ifndef MEASURE_IA_AMD_H
+#define MEASURE_IA_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MIA_ABI_VERSION 1
+
+enum mia_status {
+    MIA_OK = 0,
+    MIA_E_ARG = -1,      /* invalid argument or inconsistent sizes            */
+    MIA_E_HIP = -2,      /* a HIP runtime call failed                          */
+    MIA_E_NOMEM = -3,    /* device allocation failed                           */
+    MIA_E_RANGE = -4,    /* coordinates outside [0, boxsize] for a periodic box */
+    MIA_E_UNSUPPORTED = -5
+};
+
+/* Binning of the counter.
+ *   MIA_RPPI: (r_p, pi) grid of _measure_xi_rp_pi_box_* (measure_w_box_jk.py:157-211)
+ *   MIA_RMU : (r, mu_r) grid of _measure_xi_r_mur_box_* (measure_m_box_jk.py:164-223) */
+enum mia_mode { MIA_RPPI = 0, MIA_RMU = 1 };
+
+/* Counting method.
+ *   MIA_PLAIN: one thread per shape, FP64 geometry in the reference's op
+ *              order, global-atomic histograms (the straightforward GPU form).
+ *   MIA_FAST : cell-pair work items sorted by jackknife region pair,
+ *              per-wavefront LDS histograms in fixed point, FP32 shape
+ *              geometry with FP64 separations and FP64 edge rechecks. */
+enum mia_method { MIA_PLAIN = 0, MIA_FAST = 1 };
+
+typedef struct mia_ctx mia_ctx;
+
+/* One galaxy sample: the arrays a box counter prepares on the host
+ * (measure_w_box_jk.py:97-139).  All pointers are device pointers. */
+typedef struct mia_sample {
+    const double *pos;      /* n_pos x 3 positions, row-major, columns x,y,z        */
+    const double *pos_w;    /* n_pos position weights, or NULL for unit weights     */
+    const double *shp;      /* n_shape x 3 shape-sample positions                   */
+    const double *shp_w;    /* n_shape shape weights, or NULL for unit weights      */
+    const double *axis;     /* n_shape x 2 normalised projected axis direction      */
+    const double *e;        /* n_shape ellipticity |e| (from q, host-side)          */
+    int64_t n_pos;
+    int64_t n_shape;
+    double boxsize;         /* L (periodic box [0, L)^3)                            */
+    double jk_sub;          /* L_sub = 2*L_0p5/L_subboxes (measure_IA_base.py:428)  */
+    int32_t los;            /* line-of-sight column, 0..2                           */
+    int32_t periodic;       /* must be 1 (minimum-image wrap, measure_w_box_jk.py:160-162) */
+    int32_t jk_per_side;    /* L_subboxes; 0 = no jackknife                         */
+    int32_t reserved;
+} mia_sample;
+
+/* Bin thresholds, bisected on the host so that the device reproduces the
+ * reference's np.log10/np.floor bin index bit-for-bit.  Host pointers.
+ *   r2_edges[n_r+1]: squared-separation thresholds [S_lo, T_1..T_{n_r-1}, S_hi]:
+ *                    in domain iff S_lo <= s < S_hi, bin = #{k>=1 : T_k <= s}.
+ *   MIA_RPPI: x2_edges[n_2+1] = [pi_lo, P_1..P_{n_2-1}, pi_hi]: in domain iff
+ *                    pi_lo <= pi < pi_hi, bin = #{k>=1 : P_k <= pi}.
+ *   MIA_RMU : x2_edges[n_2]   = [M_0, M_1..M_{n_2-1}] on mu = pi/r:
+ *                    bin = n_2-1 if mu < M_0 (numpy's index -1), else
+ *                    #{k>=1 : M_k <= mu}.  Domain also needs r_p^2 > 0.   */
+typedef struct mia_bins {
+    int32_t mode;           /* enum mia_mode                                          */
+    int32_t n_r;
+    int32_t n_2;            /* pi bins or mu bins                                     */
+    int32_t accumulate_variance; /* 1: also sum (w_p w_s e+)^2 (brute-path _sigmasq) */
+    const double *r2_edges;
+    const double *x2_edges;
+} mia_bins;
+
+/* Outputs, device pointers, zeroed and filled by the count call.  With
+ * K = jk_per_side^3 (0 without jackknife) and B = n_r*n_2:
+ *   npairs [(1+K)*B] exact integer pair counts: [0] all pairs, [1+i] the
+ *                    pairs with the shape or the position in region i
+ *                    (each pair once per distinct region it touches).
+ *   dd     [(1+K)*B] the same tallies of w_p*w_s (DD); may be NULL when both
+ *                    weight arrays are NULL (DD == npairs then).
+ *   splus  [(1+K)*B] the same tallies of w_p*w_s*e+ (S+D times 2R).
+ *   scross [B]       sum of w_p*w_s*ex (SxD times 2R).
+ *   variance [B]     sum of (w_p*w_s*e+)^2, or NULL.
+ *   region_stats [K*4] per region: #positions, #shapes, sum w_s,
+ *                    sum w_s*(1-e^2/2); or NULL.                              */
+typedef struct mia_counts {
+    uint64_t *npairs;
+    double *dd;
+    double *splus;
+    double *scross;
+    double *variance;
+    double *region_stats;
+} mia_counts;
+
+/* Optional tuning (NULL = automatic).  Zero fields mean "choose". */
+typedef struct mia_tuning {
+    int32_t columns_per_side;   /* projected-plane cells per box side                 */
+    int32_t slabs_per_side;     /* line-of-sight slabs per box side                   */
+    int32_t slabs_per_block;    /* shape-block height in slabs                        */
+    int32_t block_shapes;       /* shapes per block, <= 64                            */
+    int32_t item_positions;     /* max positions per work item                        */
+    int32_t waves;              /* persistent waves of the pair kernel                */
+} mia_tuning;
+
+/* Per-call statistics, filled when the pointer is non-NULL. */
+typedef struct mia_stats {
+    double ms_prepare;      /* region ids + cell sort                                  */
+    double ms_plan;         /* shape blocks + work items (MIA_FAST)                    */
+    double ms_pairs;        /* pair kernel                                             */
+    double ms_finalize;     /* fixed point -> double                                   */
+    int64_t candidates;     /* (shape, position) pairs evaluated                       */
+    int64_t items;          /* work items (MIA_FAST)                                   */
+    int64_t rechecks;       /* pairs re-evaluated in FP64 near an edge (MIA_FAST)      */
+    int32_t columns_per_side, slabs_per_side, slabs_per_block, block_shapes;
+    int32_t waves, reserved;
+} mia_stats;
+
+int mia_abi_version(void);
+
+/* Create a context bound to HIP device `device`. */
+int mia_create(int32_t device, mia_ctx **out);
+void mia_destroy(mia_ctx *ctx);
+const char *mia_last_error(const mia_ctx *ctx);
+
+/* Count one sample: replaces the pair loop of
+ * _measure_xi_rp_pi_box_jk_{brute,tree,multiprocessing} (measure_w_box_jk.py:69-857),
+ * _measure_xi_r_mur_box_jk_* (measure_m_box_jk.py:72-898) and their no-jackknife
+ * twins (measure_w_box.py:65-656, measure_m_box.py:255-890). */
+int mia_count(mia_ctx *ctx, const mia_sample *sample, const mia_bins *bins,
+              const mia_counts *out, int32_t method, const mia_tuning *tuning,
+              mia_stats *stats, void *stream);
+
+/* Count n samples of uneven size in one persistent launch (MIA_FAST only);
+ * bins are shared, outputs are per sample.  The reference has no batched
+ * form; a caller loops measure_xi_w over samples (measure_IA.py:68-163). */
+int mia_count_batch(mia_ctx *ctx, int32_t n, const mia_sample *samples, const mia_bins *bins,
+                    const mia_counts *outs, const mia_tuning *tuning, mia_stats *stats,
+                    void *stream);
+
+/* Jackknife region id of each object (measure_IA_base.py:404-452): strict
+ * i*L_sub < x < (i+1)*L_sub on every axis, region i*L^2 + j*L + k, else 0.
+ * xyz device n x 3, region device n int32. */
+int mia_region_ids(mia_ctx *ctx, const double *xyz, int64_t n, double jk_sub,
+                   int32_t jk_per_side, int32_t *region, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MEASURE_IA_AMD_H */

@@ -1,0 +1,221 @@
+// The C ABI of include/measure_ia_amd.h: argument checks, dispatch, errors.
+#include "mia_common.hpp"
+
+#include <chrono>
+#include <cmath>
+
+struct mia_ctx {
+    mia::Ctx c;
+};
+
+namespace {
+
+thread_local std::string g_create_error;
+
+__global__ void k_u64_to_f64(const uint64_t *in, double *out, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (double)in[i];
+}
+
+void check_sample(const mia_sample *s) {
+    MIA_REQUIRE(s != nullptr, "sample is NULL");
+    MIA_REQUIRE(s->n_pos >= 0 && s->n_shape >= 0, "negative sample size");
+    MIA_REQUIRE(s->n_pos < (1ll << 31) && s->n_shape < (1ll << 31), "sample larger than 2^31 objects");
+    MIA_REQUIRE(s->n_pos == 0 || s->pos != nullptr, "pos is NULL");
+    MIA_REQUIRE(s->n_shape == 0 || (s->shp && s->axis && s->e), "shape arrays are NULL");
+    MIA_REQUIRE(s->boxsize > 0 && std::isfinite(s->boxsize), "boxsize must be positive");
+    MIA_REQUIRE(s->los >= 0 && s->los <= 2, "los must be 0, 1 or 2");
+    MIA_REQUIRE(s->jk_per_side >= 0 && s->jk_per_side <= 64, "jk_per_side out of range");
+    MIA_REQUIRE(s->jk_per_side == 0 || s->jk_sub > 0, "jk_sub must be positive with jackknife");
+    if (s->periodic != 1)
+        throw mia::Error{MIA_E_UNSUPPORTED, "only periodic boxes are supported by the device path"};
+}
+
+void check_bins(const mia_bins *b) {
+    MIA_REQUIRE(b != nullptr, "bins is NULL");
+    MIA_REQUIRE(b->mode == MIA_RPPI || b->mode == MIA_RMU, "unknown bin mode");
+    MIA_REQUIRE(b->n_r >= 1 && b->n_r <= mia::kMaxBinsR, "n_r out of range");
+    MIA_REQUIRE(b->n_2 >= 1 && b->n_2 <= mia::kMaxBins2, "n_2 out of range");
+    MIA_REQUIRE(b->r2_edges && b->x2_edges, "bin edges are NULL");
+    for (int k = 0; k < b->n_r; ++k)
+        MIA_REQUIRE(b->r2_edges[k] <= b->r2_edges[k + 1], "r2_edges not sorted");
+    MIA_REQUIRE(b->r2_edges[0] > 0, "r2_edges[0] must be positive");
+    int nx = b->mode == MIA_RPPI ? b->n_2 + 1 : b->n_2;
+    for (int k = 0; k + 1 < nx; ++k)
+        MIA_REQUIRE(b->x2_edges[k] <= b->x2_edges[k + 1], "x2_edges not sorted");
+}
+
+void check_out(const mia_counts *o, bool weighted) {
+    MIA_REQUIRE(o != nullptr, "counts is NULL");
+    MIA_REQUIRE(o->npairs && o->splus && o->scross, "npairs/splus/scross are required");
+    MIA_REQUIRE(!weighted || o->dd, "dd is required with weights");
+}
+
+template <class F>
+int guard(mia_ctx *ctx, F &&f) {
+    if (!ctx) return MIA_E_ARG;
+    try {
+        f();
+        ctx->c.last_error.clear();
+        return MIA_OK;
+    } catch (const mia::Error &e) {
+        ctx->c.last_error = e.msg;
+        return e.code;
+    } catch (const std::exception &e) {
+        ctx->c.last_error = e.what();
+        return MIA_E_HIP;
+    }
+}
+
+void unit_dd(const mia_counts &o, bool weighted, int64_t nb, hipStream_t st) {
+    if (!weighted && o.dd && nb > 0) {
+        hipLaunchKernelGGL(k_u64_to_f64, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st,
+                           (const uint64_t *)o.npairs, o.dd, nb);
+        MIA_HIP(hipGetLastError());
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int mia_abi_version(void) { return MIA_ABI_VERSION; }
+
+int mia_create(int32_t device, mia_ctx **out) {
+    if (!out) return MIA_E_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) {
+        g_create_error = "no such HIP device";
+        return MIA_E_ARG;
+    }
+    if (hipSetDevice(device) != hipSuccess) return MIA_E_HIP;
+    mia_ctx *c = new (std::nothrow) mia_ctx();
+    if (!c) return MIA_E_NOMEM;
+    c->c.device = device;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
+        cus > 0)
+        c->c.cu_count = cus;
+    *out = c;
+    return MIA_OK;
+}
+
+void mia_destroy(mia_ctx *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->c.device);
+    delete ctx;
+}
+
+const char *mia_last_error(const mia_ctx *ctx) {
+    if (!ctx) return g_create_error.c_str();
+    return ctx->c.last_error.c_str();
+}
+
+int mia_count(mia_ctx *ctx, const mia_sample *sample, const mia_bins *bins, const mia_counts *out,
+              int32_t method, const mia_tuning *tuning, mia_stats *stats, void *stream) {
+    return guard(ctx, [&] {
+        check_sample(sample);
+        check_bins(bins);
+        bool weighted = sample->pos_w != nullptr || sample->shp_w != nullptr;
+        check_out(out, weighted);
+        MIA_REQUIRE(method == MIA_PLAIN || method == MIA_FAST, "unknown method");
+        MIA_HIP(hipSetDevice(ctx->c.device));
+        hipStream_t st = (hipStream_t)stream;
+        if (stats) std::memset(stats, 0, sizeof(*stats));
+        mia::Grid g = mia::make_grid(*sample, *bins, tuning);
+        mia::Sorted pos, shp;
+        mia::Sums sums;
+        hipEvent_t e0, e1;
+        MIA_HIP(hipEventCreate(&e0));
+        MIA_HIP(hipEventCreate(&e1));
+        MIA_HIP(hipEventRecord(e0, st));
+        mia::prepare_sample(ctx->c, *sample, g, pos, shp, out->region_stats, sums, st, "s0");
+        MIA_HIP(hipEventRecord(e1, st));
+        MIA_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        MIA_HIP(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        int K = g.jk > 0 ? g.jk * g.jk * g.jk : 0;
+        int64_t nb = (int64_t)(1 + K) * bins->n_r * bins->n_2;
+        if (method == MIA_PLAIN) {
+            mia::count_plain(ctx->c, *sample, *bins, g, pos, shp, *out, weighted, st, stats);
+        } else {
+            std::vector<mia::FastSample> v(1);
+            v[0].s = sample;
+            v[0].g = g;
+            v[0].pos = pos;
+            v[0].shp = shp;
+            v[0].sums = sums;
+            v[0].out = out;
+            v[0].weighted = weighted;
+            mia::count_fast(ctx->c, v, *bins, tuning, st, stats);
+        }
+        unit_dd(*out, weighted, nb, st);
+        MIA_HIP(hipStreamSynchronize(st));
+        if (stats) {
+            stats->ms_prepare = ms;
+            stats->columns_per_side = g.nc;
+            stats->slabs_per_side = g.nz;
+        }
+    });
+}
+
+int mia_count_batch(mia_ctx *ctx, int32_t n, const mia_sample *samples, const mia_bins *bins,
+                    const mia_counts *outs, const mia_tuning *tuning, mia_stats *stats,
+                    void *stream) {
+    return guard(ctx, [&] {
+        MIA_REQUIRE(n >= 1 && samples && outs, "empty batch");
+        check_bins(bins);
+        MIA_HIP(hipSetDevice(ctx->c.device));
+        hipStream_t st = (hipStream_t)stream;
+        if (stats) std::memset(stats, 0, sizeof(*stats));
+        std::vector<mia::FastSample> v(n);
+        hipEvent_t e0, e1;
+        MIA_HIP(hipEventCreate(&e0));
+        MIA_HIP(hipEventCreate(&e1));
+        MIA_HIP(hipEventRecord(e0, st));
+        for (int i = 0; i < n; ++i) {
+            check_sample(&samples[i]);
+            bool weighted = samples[i].pos_w != nullptr || samples[i].shp_w != nullptr;
+            check_out(&outs[i], weighted);
+            v[i].s = &samples[i];
+            v[i].g = mia::make_grid(samples[i], *bins, tuning);
+            v[i].out = &outs[i];
+            v[i].weighted = weighted;
+            mia::prepare_sample(ctx->c, samples[i], v[i].g, v[i].pos, v[i].shp,
+                                outs[i].region_stats, v[i].sums, st, "s" + std::to_string(i));
+        }
+        MIA_HIP(hipEventRecord(e1, st));
+        MIA_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        MIA_HIP(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        mia::count_fast(ctx->c, v, *bins, tuning, st, stats);
+        for (int i = 0; i < n; ++i) {
+            int K = v[i].g.jk > 0 ? v[i].g.jk * v[i].g.jk * v[i].g.jk : 0;
+            unit_dd(outs[i], v[i].weighted, (int64_t)(1 + K) * bins->n_r * bins->n_2, st);
+        }
+        MIA_HIP(hipStreamSynchronize(st));
+        if (stats) {
+            stats->ms_prepare = ms;
+            stats->columns_per_side = v[0].g.nc;
+            stats->slabs_per_side = v[0].g.nz;
+        }
+    });
+}
+
+int mia_region_ids(mia_ctx *ctx, const double *xyz, int64_t n, double jk_sub, int32_t jk_per_side,
+                   int32_t *region, void *stream) {
+    return guard(ctx, [&] {
+        MIA_REQUIRE(n >= 0 && (n == 0 || (xyz && region)), "bad arrays");
+        MIA_REQUIRE(jk_per_side >= 1 && jk_sub > 0, "bad jackknife split");
+        MIA_HIP(hipSetDevice(ctx->c.device));
+        mia::region_ids(ctx->c, xyz, n, jk_sub, jk_per_side, region, (hipStream_t)stream);
+        MIA_HIP(hipStreamSynchronize((hipStream_t)stream));
+    });
+}
+
+}  // extern "C"

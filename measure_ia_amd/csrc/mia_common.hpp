@@ -1,0 +1,170 @@
+// Shared declarations of the MI355X pair-counting engine (internal).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/measure_ia_amd.h"
+
+namespace mia {
+
+constexpr int kWave = 64;
+constexpr int kMaxBins2 = 256;      // max n_2 (pi or mu bins)
+constexpr int kMaxBinsR = 256;      // max n_r
+constexpr int kMaxBlockShapes = 64;
+
+// Error carrying a MIA_E_* code; caught at the C boundary.
+struct Error {
+    int code;
+    std::string msg;
+};
+
+#define MIA_HIP(call)                                                                       \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            throw ::mia::Error{MIA_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)}; \
+    } while (0)
+
+#define MIA_REQUIRE(cond, msg)                                     \
+    do {                                                           \
+        if (!(cond)) throw ::mia::Error{MIA_E_ARG, std::string(msg)}; \
+    } while (0)
+
+// Growable named device buffers owned by a context.
+class Arena {
+  public:
+    ~Arena() { release(); }
+    void *get(const std::string &name, size_t bytes) {
+        auto it = bufs_.find(name);
+        if (it != bufs_.end() && it->second.bytes >= bytes) return it->second.ptr;
+        if (it != bufs_.end()) {
+            (void)hipFree(it->second.ptr);
+            bufs_.erase(it);
+        }
+        void *p = nullptr;
+        size_t b = bytes < 256 ? 256 : bytes;
+        if (hipMalloc(&p, b) != hipSuccess) throw Error{MIA_E_NOMEM, "hipMalloc(" + name + ")"};
+        bufs_[name] = {p, b};
+        return p;
+    }
+    template <class T>
+    T *get(const std::string &name, size_t count) {
+        return static_cast<T *>(get(name, count * sizeof(T)));
+    }
+    void release() {
+        for (auto &kv : bufs_) (void)hipFree(kv.second.ptr);
+        bufs_.clear();
+    }
+
+  private:
+    struct Buf {
+        void *ptr;
+        size_t bytes;
+    };
+    std::map<std::string, Buf> bufs_;
+};
+
+// Grid of the counting sort: columns in the projected plane (axes a, b =
+// the two non-LOS columns in ascending order) and slabs along the LOS axis l.
+// With jackknife, cell boundaries are aligned to the sub-box boundaries.
+struct Grid {
+    double L = 0, half = 0;
+    int los = 2, ax_a = 0, ax_b = 1;
+    int jk = 0;            // sub-boxes per side (0: none)
+    double jk_sub = 0;     // L_sub
+    int nc = 1, nz = 1;    // columns / slabs per side
+    int ncj = 1, nzj = 1;  // per jackknife sub-box (nc = jk*ncj when jk > 0)
+    double cw = 0, hz = 0; // nominal column width and slab height
+    double margin = 0;     // geometric slack for rounding at cell edges
+    int keys() const { return nc * nc * (nz + 1); }
+};
+
+// Sorted sample in HBM (structure of arrays, sorted by cell key).
+struct Sorted {
+    int64_t n = 0;
+    double *a = nullptr, *b = nullptr, *l = nullptr;  // coordinates in (a, b, los) order
+    double *w = nullptr;                              // weights (1 when absent)
+    int32_t *reg = nullptr;                           // jackknife region (0 without jk)
+    int32_t *key = nullptr;                           // cell key
+    double *ax = nullptr, *ay = nullptr, *e = nullptr; // shapes only
+    int32_t *start = nullptr;                         // keys+1 offsets
+};
+
+struct Sums {
+    double wmax_pos = 1, wmax_shape = 1;
+    int out_of_range = 0;
+};
+
+struct Ctx;
+
+// Host-side helpers implemented in the .hip files.
+void region_ids(Ctx &ctx, const double *xyz, int64_t n, double jk_sub, int jk, int32_t *region,
+                hipStream_t st);
+void prepare_sample(Ctx &ctx, const mia_sample &s, Grid &g, Sorted &pos, Sorted &shp,
+                    double *region_stats, Sums &sums, hipStream_t st, const std::string &tag);
+void count_plain(Ctx &ctx, const mia_sample &s, const mia_bins &bins, const Grid &g,
+                 const Sorted &pos, const Sorted &shp, const mia_counts &out, bool weighted,
+                 hipStream_t st, mia_stats *stats);
+
+struct FastSample {
+    const mia_sample *s;
+    Grid g;
+    Sorted pos, shp;
+    Sums sums;
+    const mia_counts *out;
+    bool weighted;
+};
+void count_fast(Ctx &ctx, std::vector<FastSample> &samples, const mia_bins &bins,
+                const mia_tuning *tuning, hipStream_t st, mia_stats *stats);
+
+struct Ctx {
+    int device = 0;
+    std::string last_error;
+    Arena arena;
+    int cu_count = 256;
+};
+
+template <class T>
+void fill(T *p, T v, int64_t n, hipStream_t st);
+template <class T>
+void exclusive_scan(Ctx &ctx, const T *in, T *out, int64_t n, hipStream_t st,
+                    const std::string &tag, int depth);
+
+// Pick columns/slabs per side for a sample (autotuned defaults).
+Grid make_grid(const mia_sample &s, const mia_bins &bins, const mia_tuning *t);
+
+// ---------------------------------------------------------------------------
+// device helpers
+
+__device__ __forceinline__ int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
+
+// Index i with the reference's strict test i*L_sub < x < (i+1)*L_sub, or -1.
+__device__ __forceinline__ int strict_axis_index(double x, double Lsub, int jk) {
+    if (!(x > 0.0) || !(x < Lsub * jk + Lsub)) return -1;
+    double q = x / Lsub;
+    int i0 = (int)floor(q);
+    for (int i = i0 - 1; i <= i0 + 1; ++i) {
+        if (i < 0 || i >= jk) continue;
+        double lo = (double)i * Lsub;
+        double hi = (double)(i + 1) * Lsub;
+        if (x > lo && x < hi) return i;
+    }
+    return -1;
+}
+
+// The reference's minimum-image wrap (measure_w_box_jk.py:160-162).
+__device__ __forceinline__ double wrap_sep(double d, double L, double half) {
+    if (d > half) d -= L;
+    if (d < -half) d += L;
+    return d;
+}
+
+}  // namespace mia

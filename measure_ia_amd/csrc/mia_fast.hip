@@ -1,0 +1,812 @@
+// Fast pair counter: cell-pair work items, per-wavefront LDS histograms.
+//
+// Replaces the per-shape KDTree loop and np.add.at scatter of the reference
+// (src/measureia/measure_w_box_jk.py:386-461, measure_m_box_jk.py:403-487)
+// with three steps on the device:
+//   1. plan   : shapes of one column and a few line-of-sight slabs form a
+//               block (<= 64 shapes, one jackknife region R_s); every block
+//               is paired with the position cell ranges within reach, split
+//               at jackknife sub-box faces so that every work item has one
+//               position region R_p and one periodic shift per axis.  Items
+//               are counting-sorted by (sample, R_s, R_p).
+//   2. pairs  : a persistent grid of wavefronts takes contiguous runs of the
+//               sorted list, balanced by pair count.  Each wave caches its
+//               shape block in LDS and streams positions through registers,
+//               64 at a time.  Separations are the reference's own FP64
+//               differences, so r_p^2 (or r^2) and pi are compared exactly
+//               with the host-bisected thresholds (an FP32 log2 only guesses
+//               the bin); e+ and ex are FP64 (CDNA4 runs FP64 FMA at the rate
+//               of non-packed FP32) and go into the wave's private LDS
+//               histogram with LDS atomics.  mu = pi/r is screened in FP32 and recomputed in
+//               FP64 when it lies within a guard band of a bin edge; the
+//               arccos NaN rule is likewise checked in FP64 for nearly
+//               parallel pairs.
+//   3. flush  : a wave adds its histogram to the global FP64 accumulators of
+//               the total and of tally[R_s] / tally[R_p] only when the
+//               (R_s, R_p) key changes, i.e. a handful of times per wave.
+// Pair counts are integers (exact, order independent); the e+ sums are FP64
+// throughout, so every bin carries only the rounding of FP64 additions.
+#include "mia_common.hpp"
+
+#include <cmath>
+
+namespace mia {
+
+namespace {
+
+constexpr int kWavesPerBlock = 4;
+constexpr int kThreads = kWave * kWavesPerBlock;
+constexpr float kMuBand = 2e-6f;     // |mu32 - mu64| bound with margin
+constexpr float kSin2Exact = 1e-10f; // FP64 arccos-domain check below this sin^2
+
+struct Block {
+    int32_t s0, ns;
+    int32_t col, grp;   // column index, slab group (== ngroups: tail)
+    int32_t sample, rs;
+};
+
+struct Item {
+    int32_t blk;
+    int32_t p0;
+    int32_t np;
+    int16_t rp;
+    uint8_t shifts;     // 2 bits per axis (a, b, l): 0 -> 0, 1 -> +L, 2 -> -L ; bit 6: per-pair wrap
+    uint8_t pad;
+};
+static_assert(sizeof(Item) == 16, "Item layout");
+
+struct SampleDev {
+    // sorted sample
+    const double *pa, *pb, *pl, *pw;
+    const double *sa, *sb, *sl, *sax, *say, *se, *sw;
+    const int32_t *pstart;
+    double L, half;
+    int los, K, nbins, pad;
+    // accumulators (the caller's outputs)
+    unsigned long long *npairs;   // (1+K)*B
+    double *dd, *sp;              // (1+K)*B
+    double *sx, *var;             // B
+};
+
+struct PlanArgs {
+    Grid g;
+    int mode, periodic, sample, K1;
+    int ng, G;           // slab groups per column, slabs per group
+    int kx, nxw;         // stencil half width / width in columns
+    int pmax;
+    double reach_r, reach_z;
+    const int32_t *sstart, *pstart;
+    const Block *blocks;
+    int nblocks;
+    int blk_base;        // global index of this sample's first block
+    int32_t *key_count;  // count pass
+    int32_t *key_cursor; // emit pass
+    const int32_t *key_off;
+    Item *items;
+    int emit;
+};
+
+__device__ __forceinline__ int region_of(const Grid &g, int ia, int ib, int il) {
+    if (g.jk <= 0) return 0;
+    int j[3];
+    j[g.ax_a] = ia / g.ncj;
+    j[g.ax_b] = ib / g.ncj;
+    j[g.los] = il / g.nzj;
+    return j[0] * g.jk * g.jk + j[1] * g.jk + j[2];
+}
+
+// Separation range [lo, hi] (unwrapped) -> minimum |minimum-image value| and
+// the shift class: 0 none, 1 +L for all, 2 -L for all, 3 mixed.
+__device__ __forceinline__ void axis_class(double lo, double hi, double L, double half,
+                                           bool periodic, double &dmin, int &cls) {
+    if (!periodic) {
+        dmin = lo > 0 ? lo : (hi < 0 ? -hi : 0.0);
+        cls = 0;
+        return;
+    }
+    dmin = 1e300;
+    // piece below -half: value + L
+    if (lo < -half) {
+        double a = lo + L, b = (hi < -half ? hi : -half) + L;
+        double d = a > 0 ? a : (b < 0 ? -b : 0.0);
+        dmin = d < dmin ? d : dmin;
+    }
+    if (hi >= -half && lo <= half) {
+        double a = lo > -half ? lo : -half, b = hi < half ? hi : half;
+        double d = a > 0 ? a : (b < 0 ? -b : 0.0);
+        dmin = d < dmin ? d : dmin;
+    }
+    if (hi > half) {
+        double a = (lo > half ? lo : half) - L, b = hi - L;
+        double d = a > 0 ? a : (b < 0 ? -b : 0.0);
+        dmin = d < dmin ? d : dmin;
+    }
+    if (lo > half)
+        cls = 2;
+    else if (hi < -half)
+        cls = 1;
+    else if (lo >= -half && hi <= half)
+        cls = 0;
+    else
+        cls = 3;
+}
+
+// Enumerate (count or emit) the work items of one (block, stencil column).
+__global__ void k_plan_items(PlanArgs P) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t total = (int64_t)P.nblocks * P.nxw * P.nxw;
+    if (t >= total) return;
+    const Grid &g = P.g;
+    int bl = (int)(t / (P.nxw * P.nxw));
+    int so = (int)(t % (P.nxw * P.nxw));
+    int ox = so / P.nxw, oy = so % P.nxw;
+    Block B = P.blocks[bl];
+    int ia = B.col / g.nc, ib = B.col % g.nc;
+    bool full = P.nxw >= g.nc;
+    int jx = full ? ox : ia - P.kx + ox;
+    int jy = full ? oy : ib - P.kx + oy;
+    if (!full) {
+        if (P.periodic) {
+            jx = ((jx % g.nc) + g.nc) % g.nc;
+            jy = ((jy % g.nc) + g.nc) % g.nc;
+        } else if (jx < 0 || jx >= g.nc || jy < 0 || jy >= g.nc) {
+            return;
+        }
+    }
+    double m = g.margin;
+    double da_min, db_min;
+    int ca, cb;
+    axis_class(ia * g.cw - (jx + 1) * g.cw - 2 * m, (ia + 1) * g.cw - jx * g.cw + 2 * m, g.L, g.half,
+               P.periodic, da_min, ca);
+    axis_class(ib * g.cw - (jy + 1) * g.cw - 2 * m, (ib + 1) * g.cw - jy * g.cw + 2 * m, g.L, g.half,
+               P.periodic, db_min, cb);
+    double rr = P.reach_r + 4 * m;
+    double dxy2 = da_min * da_min + db_min * db_min;
+    if (dxy2 >= rr * rr) return;
+    double zlo, zhi;
+    bool tail = B.grp >= P.ng;
+    if (tail) {
+        zlo = 0.0;
+        zhi = g.L;
+    } else {
+        zlo = (double)(B.grp * P.G) * g.hz;
+        zhi = (double)((B.grp + 1) * P.G) * g.hz;
+    }
+    int col = jx * g.nc + jy;
+    int64_t base = (int64_t)col * (g.nz + 1);
+    int rs = B.rs;
+    int key_base = (B.sample * P.K1 + rs) * P.K1;
+    // walk slabs, merging consecutive included slabs with equal (shift class, jk slab)
+    int run_k0 = -1, run_cls = 0, run_jz = 0;
+    auto emit_range = [&](int p_begin, int p_end, int rp, int cz) {
+        if (p_end <= p_begin) return;
+        bool wrap = (ca == 3) || (cb == 3) || (cz == 3);
+        uint8_t sh = (uint8_t)((ca == 3 ? 0 : ca) | ((cb == 3 ? 0 : cb) << 2) |
+                               ((cz == 3 ? 0 : cz) << 4) | (wrap ? 64 : 0));
+        int key = key_base + rp;
+        for (int p = p_begin; p < p_end; p += P.pmax) {
+            int np = p_end - p < P.pmax ? p_end - p : P.pmax;
+            if (!P.emit) {
+                atomicAdd(&P.key_count[key], 1);
+            } else {
+                int slot = P.key_off[key] + atomicAdd(&P.key_cursor[key], 1);
+                Item it;
+                it.blk = P.blk_base + bl;
+                it.p0 = p;
+                it.np = np;
+                it.rp = (int16_t)rp;
+                it.shifts = sh;
+                it.pad = 0;
+                P.items[slot] = it;
+            }
+        }
+    };
+    double rz = P.reach_z + 4 * m;
+    for (int k = 0; k <= g.nz; ++k) {
+        bool inc = false;
+        int cz = 0;
+        if (k < g.nz) {
+            double dzmin;
+            axis_class(zlo - (k + 1) * g.hz - 2 * m, zhi - k * g.hz + 2 * m, g.L, g.half, P.periodic,
+                       dzmin, cz);
+            inc = dzmin < rz;
+            if (inc && P.mode == MIA_RMU) inc = dxy2 + dzmin * dzmin < rr * rr;
+        }
+        int jz = k / g.nzj;
+        if (run_k0 >= 0 && (!inc || cz != run_cls || jz != run_jz)) {
+            // close run [run_k0, k)
+            int rp = region_of(g, jx, jy, run_k0);
+            emit_range(P.pstart[base + run_k0], P.pstart[base + k], rp, run_cls);
+            run_k0 = -1;
+        }
+        if (inc && run_k0 < 0) {
+            run_k0 = k;
+            run_cls = cz;
+            run_jz = jz;
+        }
+    }
+    // anomalous tail of the target column: region 0, any line-of-sight value
+    emit_range(P.pstart[base + g.nz], P.pstart[base + g.nz + 1], 0, 3);
+}
+
+__global__ void k_blocks_count(const int32_t *sstart, Grid g, int ng, int G, int bsz,
+                               int32_t *nblk) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int ncol = g.nc * g.nc;
+    if (t >= ncol * (ng + 1)) return;
+    int col = t / (ng + 1), grp = t % (ng + 1);
+    int64_t base = (int64_t)col * (g.nz + 1);
+    int c = grp < ng ? sstart[base + (grp + 1) * G] - sstart[base + grp * G]
+                     : sstart[base + g.nz + 1] - sstart[base + g.nz];
+    nblk[t] = (c + bsz - 1) / bsz;
+}
+
+__global__ void k_blocks_emit(const int32_t *sstart, Grid g, int ng, int G, int bsz,
+                              const int32_t *off, int sample, Block *blocks) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int ncol = g.nc * g.nc;
+    if (t >= ncol * (ng + 1)) return;
+    int col = t / (ng + 1), grp = t % (ng + 1);
+    int64_t base = (int64_t)col * (g.nz + 1);
+    int s0, c, rs;
+    if (grp < ng) {
+        s0 = sstart[base + grp * G];
+        c = sstart[base + (grp + 1) * G] - s0;
+        rs = region_of(g, col / g.nc, col % g.nc, grp * G);
+    } else {
+        s0 = sstart[base + g.nz];
+        c = sstart[base + g.nz + 1] - s0;
+        rs = 0;
+    }
+    int o = off[t];
+    for (int b = 0; b * bsz < c; ++b) {
+        Block B;
+        B.s0 = s0 + b * bsz;
+        B.ns = c - b * bsz < bsz ? c - b * bsz : bsz;
+        B.col = col;
+        B.grp = grp;
+        B.sample = sample;
+        B.rs = rs;
+        blocks[o + b] = B;
+    }
+}
+
+__global__ void k_item_cost(const Item *items, const Block *blocks, int64_t n, int64_t *cost) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Item it = items[i];
+    cost[i] = (int64_t)blocks[it.blk].ns * it.np;
+}
+
+// ---------------------------------------------------------------------------
+// pair kernel
+
+struct PairArgs {
+    const Item *items;
+    const Block *blocks;
+    const int64_t *cost_prefix;  // exclusive, n_items + 1
+    int64_t n_items;
+    int64_t total_cost;
+    int n_waves;
+    int mode, n_r, n_2, weighted, var, nbins;
+    const double *T, *X;
+    float lg0, lgs;              // log2(T[0]), n_r / log2(T[n_r]/T[0])
+    float x0, xs;                // pi: X[0], n_2/(X[n_2]-X[0]);  mu: -1, n_2/2
+    const SampleDev *samples;
+    int hist_stride;             // bytes per wave histogram
+    int off_sp, off_sx, off_dd, off_var;  // byte offsets inside a wave histogram
+    unsigned long long *rechecks;
+};
+
+struct ShapeLds {
+    double a, b, l;
+    double ax, ay, ew, w;
+    double pad;
+};
+
+__device__ void flush_hist(const PairArgs &P, char *hist, const SampleDev &S, int rs, int rp,
+                           int lane) {
+    uint32_t *cnt = (uint32_t *)hist;
+    double *sp = (double *)(hist + P.off_sp);
+    double *sx = (double *)(hist + P.off_sx);
+    double *dd = P.weighted ? (double *)(hist + P.off_dd) : nullptr;
+    double *vr = P.var ? (double *)(hist + P.off_var) : nullptr;
+    const int B = P.nbins;
+    for (int b = lane; b < B; b += kWave) {
+        uint32_t c = cnt[b];
+        if (c == 0) continue;
+        double vsp = sp[b], vsx = sx[b];
+        double vdd = dd ? dd[b] : 0.0;
+        double vvr = vr ? vr[b] : 0.0;
+        cnt[b] = 0;
+        sp[b] = 0.0;
+        sx[b] = 0.0;
+        if (dd) dd[b] = 0.0;
+        if (vr) vr[b] = 0.0;
+        atomicAdd(&S.npairs[b], (unsigned long long)c);
+        atomicAdd(&S.sp[b], vsp);
+        atomicAdd(&S.sx[b], vsx);
+        if (dd && S.dd) atomicAdd(&S.dd[b], vdd);
+        if (vr && S.var) atomicAdd(&S.var[b], vvr);
+        if (S.K > 0) {
+            int o = (1 + rs) * B + b;
+            atomicAdd(&S.npairs[o], (unsigned long long)c);
+            atomicAdd(&S.sp[o], vsp);
+            if (dd && S.dd) atomicAdd(&S.dd[o], vdd);
+            if (rp != rs) {
+                o = (1 + rp) * B + b;
+                atomicAdd(&S.npairs[o], (unsigned long long)c);
+                atomicAdd(&S.sp[o], vsp);
+                if (dd && S.dd) atomicAdd(&S.dd[o], vdd);
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ double shift_val(int cls, double L) {
+    return cls == 1 ? L : (cls == 2 ? -L : 0.0);
+}
+
+__device__ __forceinline__ double r2_column_order(double a2, double b2, double l2, int los) {
+    if (los == 2) return (a2 + b2) + l2;
+    if (los == 1) return (a2 + l2) + b2;
+    return (l2 + a2) + b2;
+}
+
+template <int MODE, bool WRAP>
+__device__ __forceinline__ void run_item(const PairArgs &P, const SampleDev &S, const Item &it,
+                                         const ShapeLds *sh, int ns, int s0, char *hist,
+                                         const double *T, const double *X, int lane) {
+    uint32_t *cnt = (uint32_t *)hist;
+    double *hsp = (double *)(hist + P.off_sp);
+    double *hsx = (double *)(hist + P.off_sx);
+    double *hdd = (double *)(hist + P.off_dd);
+    double *hvr = (double *)(hist + P.off_var);
+    const double L = S.L, half = S.half;
+    const double sha = WRAP ? 0.0 : shift_val(it.shifts & 3, L);
+    const double shb = WRAP ? 0.0 : shift_val((it.shifts >> 2) & 3, L);
+    const double shl = WRAP ? 0.0 : shift_val((it.shifts >> 4) & 3, L);
+    const double T0 = T[0], Tn = T[P.n_r];
+    const int n_r = P.n_r, n_2 = P.n_2;
+    for (int base = 0; base < it.np; base += kWave) {
+        int pi = base + lane;
+        bool valid = pi < it.np;
+        int p = it.p0 + (valid ? pi : 0);
+        double pa = S.pa[p], pb = S.pb[p], pl = S.pl[p];
+        double pw = S.pw[p];
+        for (int j = 0; j < ns; ++j) {
+            const ShapeLds q = sh[j];
+            // the reference's own FP64 separation (measure_w_box_jk.py:159-162)
+            double da = q.a - pa, db = q.b - pb, dl = q.l - pl;
+            if (WRAP) {
+                da = wrap_sep(da, L, half);
+                db = wrap_sep(db, L, half);
+                dl = wrap_sep(dl, L, half);
+            } else {
+                da = da + sha;
+                db = db + shb;
+                dl = dl + shl;
+            }
+            double a2 = da * da, b2 = db * db;
+            double s = a2 + b2;
+            double rv;
+            bool in;
+            if (MODE == MIA_RPPI) {
+                rv = s;
+                in = valid && s >= T0 && s < Tn && dl >= X[0] && dl < X[n_2];
+            } else {
+                rv = r2_column_order(a2, b2, dl * dl, S.los);
+                in = valid && s > 0.0 && rv >= T0 && rv < Tn;
+            }
+            if (!__any(in)) continue;
+            if (!in) continue;
+            // r bin: FP32 log2 guess, exact FP64 correction against the thresholds
+            float rv32 = (float)rv;
+            float gr = (__log2f(rv32) - P.lg0) * P.lgs;
+            gr = fminf(fmaxf(gr, 0.0f), (float)(n_r - 1));
+            int ir = (int)gr;
+            while (ir < n_r - 1 && rv >= T[ir + 1]) ++ir;
+            while (ir > 0 && rv < T[ir]) --ir;
+            int i2;
+            float dlf = (float)dl;
+            if (MODE == MIA_RPPI) {
+                float g2 = (dlf - P.x0) * P.xs;
+                g2 = fminf(fmaxf(g2, 0.0f), (float)(n_2 - 1));
+                i2 = (int)g2;
+                while (i2 < n_2 - 1 && dl >= X[i2 + 1]) ++i2;
+                while (i2 > 0 && dl < X[i2]) --i2;
+            } else {
+                // mu = pi / r screened in FP32; exact FP64 pi / sqrt(r^2) near an edge
+                float mu = dlf * __builtin_amdgcn_rsqf(rv32);
+                float g2 = (mu + 1.0f) * P.xs;
+                g2 = fminf(fmaxf(g2, 0.0f), (float)(n_2 - 1));
+                i2 = (int)g2;
+                float lo = (float)X[i2];
+                float hi = i2 + 1 < n_2 ? (float)X[i2 + 1] : 3.0f;
+                bool sure = (mu - kMuBand >= lo) && (mu + kMuBand < hi);
+                if (!sure) {
+                    double mu64 = dl / sqrt(rv);
+                    if (mu64 < X[0]) {
+                        i2 = n_2 - 1;  // numpy's index -1
+                    } else {
+                        while (i2 < n_2 - 1 && mu64 >= X[i2 + 1]) ++i2;
+                        while (i2 > 0 && mu64 < X[i2]) --i2;
+                    }
+                    atomicAdd(P.rechecks, 1ull);
+                }
+            }
+            // e+ = e cos 2phi = e (2 cos^2 phi - 1), ex = e sin 2phi = 2 e cos phi |sin phi|
+            // (|.| because phi = arccos(d) lies in [0, pi]); cos phi = t/r_p,
+            // |sin phi| = |c|/r_p with t, c the dot and cross products.
+            double t = fma(da, q.ax, db * q.ay);
+            double c = fma(da, q.ay, -(db * q.ax));
+            double r0 = __builtin_amdgcn_rcp(s);
+            double inv = fma(r0, fma(-s, r0, 1.0), r0);
+            double cos2 = t * t * inv;
+            double ep = q.ew * fma(2.0, cos2, -1.0);
+            double ex = 2.0 * q.ew * t * fabs(c) * inv;
+            if (c * c * inv < kSin2Exact) {
+                // nearly parallel: the reference's d may round outside [-1, 1],
+                // where arccos gives NaN and the pair adds no e+ (measure_w_box_jk.py:170-175)
+                int sidx = s0 + j;
+                double rp = sqrt(s);
+                double d = 0.0 + (da / rp) * S.sax[sidx];
+                d = d + (db / rp) * S.say[sidx];
+                if (!(fabs(d) <= 1.0)) {
+                    ep = 0.0;
+                    ex = 0.0;
+                }
+                atomicAdd(P.rechecks, 1ull);
+            }
+            int b = ir * n_2 + i2;
+            double vsp = pw * ep;
+            atomicAdd(&cnt[b], 1u);
+            atomicAdd(&hsp[b], vsp);
+            atomicAdd(&hsx[b], pw * ex);
+            if (P.weighted) atomicAdd(&hdd[b], pw * q.w);
+            if (P.var) atomicAdd(&hvr[b], vsp * vsp);
+        }
+    }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kThreads) void k_pairs(PairArgs P) {
+    extern __shared__ __align__(16) char lds[];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = threadIdx.x / kWave;
+    // thresholds (shared by the block's waves)
+    double *T = (double *)lds;
+    double *X = T + (P.n_r + 1);
+    int nx = P.n_2 + 1;
+    for (int i = threadIdx.x; i < P.n_r + 1; i += blockDim.x) T[i] = P.T[i];
+    for (int i = threadIdx.x; i < nx; i += blockDim.x) X[i] = P.X[i];
+    size_t thr_bytes = ((size_t)(P.n_r + 1 + nx) * 8 + 15) & ~(size_t)15;
+    ShapeLds *sh = (ShapeLds *)(lds + thr_bytes) + wv * kMaxBlockShapes;
+    char *hist = lds + thr_bytes + sizeof(ShapeLds) * kMaxBlockShapes * kWavesPerBlock +
+                 (size_t)wv * P.hist_stride;
+    for (int i = lane * 4; i < P.hist_stride; i += kWave * 4) *(uint32_t *)(hist + i) = 0u;
+    __syncthreads();
+
+    int w = blockIdx.x * kWavesPerBlock + wv;
+    if (w >= P.n_waves || P.n_items == 0) return;
+    // items [it0, it1) whose cost prefix falls in this wave's share
+    auto first_at_least = [&](int64_t target) {
+        int64_t lo = 0, hi = P.n_items;
+        while (lo < hi) {
+            int64_t mid = (lo + hi) >> 1;
+            // prefix[mid] * W >= target * C (any monotone rule works: the
+            // neighbouring wave evaluates the same boundary)
+            if ((double)P.cost_prefix[mid] * P.n_waves >= (double)target * (double)P.total_cost)
+                hi = mid;
+            else
+                lo = mid + 1;
+        }
+        return lo;
+    };
+    int64_t it0 = first_at_least(w);
+    int64_t it1 = first_at_least(w + 1);
+    int cur_key_s = -1, cur_rs = -1, cur_rp = -1;
+    int cur_blk = -1;
+    const SampleDev *Sp = nullptr;
+    int ns = 0, s0 = 0;
+    for (int64_t i = it0; i < it1; ++i) {
+        Item it = P.items[i];
+        Block B = P.blocks[it.blk];
+        if (B.sample != cur_key_s || B.rs != cur_rs || it.rp != cur_rp) {
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            if (Sp) flush_hist(P, hist, *Sp, cur_rs, cur_rp, lane);
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            cur_key_s = B.sample;
+            cur_rs = B.rs;
+            cur_rp = it.rp;
+            Sp = &P.samples[B.sample];
+        }
+        const SampleDev &S = *Sp;
+        if (it.blk != cur_blk) {
+            cur_blk = it.blk;
+            ns = B.ns;
+            s0 = B.s0;
+            if (lane < ns) {
+                int si = s0 + lane;
+                ShapeLds q;
+                q.a = S.sa[si];
+                q.b = S.sb[si];
+                q.l = S.sl[si];
+                q.ax = S.sax[si];
+                q.ay = S.say[si];
+                q.w = S.sw[si];
+                q.ew = S.se[si] * q.w;
+                q.pad = 0.0;
+                sh[lane] = q;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        }
+        if (it.shifts & 64)
+            run_item<MODE, true>(P, S, it, sh, ns, s0, hist, T, X, lane);
+        else
+            run_item<MODE, false>(P, S, it, sh, ns, s0, hist, T, X, lane);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    if (Sp) flush_hist(P, hist, *Sp, cur_rs, cur_rp, lane);
+}
+
+inline unsigned nblk(int64_t n, int t) { return (unsigned)((n + t - 1) / t < 1 ? 1 : (n + t - 1) / t); }
+
+}  // namespace
+
+void count_fast(Ctx &ctx, std::vector<FastSample> &samples, const mia_bins &bins,
+                const mia_tuning *tuning, hipStream_t st, mia_stats *stats) {
+    const int ns_ = (int)samples.size();
+    const int n_r = bins.n_r, n_2 = bins.n_2, B = n_r * n_2;
+    MIA_REQUIRE(n_r >= 1 && n_r <= kMaxBinsR && n_2 >= 1 && n_2 <= kMaxBins2, "bin counts out of range");
+    const double reach_r = std::sqrt(bins.r2_edges[n_r]);
+    const double reach_z = bins.mode == MIA_RPPI
+                               ? std::fmax(std::fabs(bins.x2_edges[0]), std::fabs(bins.x2_edges[n_2]))
+                               : reach_r;
+    const int nx2 = bins.mode == MIA_RPPI ? n_2 + 1 : n_2;
+    double *T = ctx.arena.get<double>("fast/T", n_r + 1);
+    double *X = ctx.arena.get<double>("fast/X", n_2 + 1);
+    std::vector<double> xh(n_2 + 1, 3.0);
+    for (int i = 0; i < nx2; ++i) xh[i] = bins.x2_edges[i];
+    MIA_HIP(hipMemcpyAsync(T, bins.r2_edges, sizeof(double) * (n_r + 1), hipMemcpyHostToDevice, st));
+    MIA_HIP(hipMemcpyAsync(X, xh.data(), sizeof(double) * (n_2 + 1), hipMemcpyHostToDevice, st));
+
+    hipEvent_t ev[4];
+    for (auto &e : ev) MIA_HIP(hipEventCreate(&e));
+    MIA_HIP(hipEventRecord(ev[0], st));
+
+    // ---- plan: shape blocks per sample
+    const int bsz = tuning && tuning->block_shapes > 0 ? std::min(tuning->block_shapes, kMaxBlockShapes) : kMaxBlockShapes;
+    const int pmax = tuning && tuning->item_positions > 0 ? tuning->item_positions : 1024;
+    std::vector<int> blk_base(ns_ + 1, 0), gsz(ns_), ngr(ns_);
+    int K1max = 1;
+    for (int si = 0; si < ns_; ++si) {
+        FastSample &F = samples[si];
+        const Grid &g = F.g;
+        K1max = std::max(K1max, g.jk > 0 ? g.jk * g.jk * g.jk : 1);
+        // slabs per block: aim for ~bsz shapes per block, dividing the per-jk slab count
+        int G = 1;
+        if (tuning && tuning->slabs_per_block > 0) {
+            G = tuning->slabs_per_block;
+        } else {
+            double per_slab = (double)F.shp.n / ((double)g.nc * g.nc * g.nz);
+            double want = per_slab > 0 ? 16.0 / per_slab : g.nzj;
+            G = (int)std::floor(want);
+        }
+        G = std::max(1, std::min(G, g.nzj));
+        while (g.nzj % G) --G;
+        gsz[si] = G;
+        ngr[si] = g.nz / G;
+        int ncol = g.nc * g.nc;
+        int64_t nt = (int64_t)ncol * (ngr[si] + 1);
+        int32_t *nb = ctx.arena.get<int32_t>("fast/nblk" + std::to_string(si), nt + 1);
+        int32_t *nbo = ctx.arena.get<int32_t>("fast/nblko" + std::to_string(si), nt + 1);
+        hipLaunchKernelGGL(k_blocks_count, dim3(nblk(nt, 256)), dim3(256), 0, st, F.shp.start, g,
+                           ngr[si], G, bsz, nb);
+        MIA_HIP(hipGetLastError());
+        exclusive_scan<int32_t>(ctx, nb, nbo, nt, st, "fast/bs" + std::to_string(si), 0);
+        int32_t tot = 0;
+        MIA_HIP(hipMemcpyAsync(&tot, nbo + nt, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        MIA_HIP(hipStreamSynchronize(st));
+        blk_base[si + 1] = blk_base[si] + tot;
+    }
+    const int nblocks = blk_base[ns_];
+    Block *blocks = ctx.arena.get<Block>("fast/blocks", (size_t)std::max(nblocks, 1));
+    for (int si = 0; si < ns_; ++si) {
+        FastSample &F = samples[si];
+        const Grid &g = F.g;
+        int64_t nt = (int64_t)g.nc * g.nc * (ngr[si] + 1);
+        int32_t *nbo = ctx.arena.get<int32_t>("fast/nblko" + std::to_string(si), nt + 1);
+        hipLaunchKernelGGL(k_blocks_emit, dim3(nblk(nt, 256)), dim3(256), 0, st, F.shp.start, g,
+                           ngr[si], gsz[si], bsz, nbo, si, blocks + blk_base[si]);
+        MIA_HIP(hipGetLastError());
+    }
+
+    // ---- plan: work items, counting-sorted by (sample, R_s, R_p)
+    const int K1 = K1max;
+    const int64_t nkeys = (int64_t)ns_ * K1 * K1;
+    int32_t *kc = ctx.arena.get<int32_t>("fast/keycount", nkeys + 1);
+    int32_t *ko = ctx.arena.get<int32_t>("fast/keyoff", nkeys + 1);
+    int32_t *kcur = ctx.arena.get<int32_t>("fast/keycur", nkeys + 1);
+    fill<int32_t>(kc, 0, nkeys + 1, st);
+    fill<int32_t>(kcur, 0, nkeys + 1, st);
+    std::vector<PlanArgs> plans(ns_);
+    for (int si = 0; si < ns_; ++si) {
+        FastSample &F = samples[si];
+        PlanArgs &P = plans[si];
+        P.g = F.g;
+        P.mode = bins.mode;
+        P.periodic = F.s->periodic;
+        P.sample = si;
+        P.K1 = K1;
+        P.ng = ngr[si];
+        P.G = gsz[si];
+        P.kx = (int)std::ceil((reach_r + 4 * F.g.margin) / F.g.cw) + 1;
+        P.nxw = std::min(F.g.nc, 2 * P.kx + 1);
+        P.pmax = pmax;
+        P.reach_r = reach_r;
+        P.reach_z = reach_z;
+        P.sstart = F.shp.start;
+        P.pstart = F.pos.start;
+        P.blocks = blocks + blk_base[si];
+        P.nblocks = blk_base[si + 1] - blk_base[si];
+        P.blk_base = blk_base[si];
+        P.key_count = kc;
+        P.key_cursor = kcur;
+        P.key_off = ko;
+        P.items = nullptr;
+        P.emit = 0;
+        int64_t nt = (int64_t)P.nblocks * P.nxw * P.nxw;
+        if (nt > 0) {
+            hipLaunchKernelGGL(k_plan_items, dim3(nblk(nt, 256)), dim3(256), 0, st, P);
+            MIA_HIP(hipGetLastError());
+        }
+    }
+    exclusive_scan<int32_t>(ctx, kc, ko, nkeys, st, "fast/keys", 0);
+    int32_t nitems32 = 0;
+    MIA_HIP(hipMemcpyAsync(&nitems32, ko + nkeys, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    MIA_HIP(hipStreamSynchronize(st));
+    const int64_t nitems = nitems32;
+    Item *items = ctx.arena.get<Item>("fast/items", (size_t)std::max<int64_t>(nitems, 1));
+    for (int si = 0; si < ns_; ++si) {
+        PlanArgs P = plans[si];
+        P.emit = 1;
+        P.items = items;
+        int64_t nt = (int64_t)P.nblocks * P.nxw * P.nxw;
+        if (nt > 0) {
+            hipLaunchKernelGGL(k_plan_items, dim3(nblk(nt, 256)), dim3(256), 0, st, P);
+            MIA_HIP(hipGetLastError());
+        }
+    }
+    int64_t *cost = ctx.arena.get<int64_t>("fast/cost", nitems + 1);
+    int64_t *cpre = ctx.arena.get<int64_t>("fast/costpre", nitems + 1);
+    if (nitems > 0) {
+        hipLaunchKernelGGL(k_item_cost, dim3(nblk(nitems, 256)), dim3(256), 0, st, items, blocks,
+                           nitems, cost);
+        MIA_HIP(hipGetLastError());
+    }
+    exclusive_scan<int64_t>(ctx, cost, cpre, nitems, st, "fast/cost", 0);
+    int64_t total_cost = 0;
+    MIA_HIP(hipMemcpyAsync(&total_cost, cpre + nitems, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    MIA_HIP(hipStreamSynchronize(st));
+    MIA_HIP(hipEventRecord(ev[1], st));
+
+    // ---- accumulators
+    const bool any_var = bins.accumulate_variance != 0;
+    bool any_w = false;
+    std::vector<SampleDev> sd(ns_);
+    for (int si = 0; si < ns_; ++si) {
+        FastSample &F = samples[si];
+        any_w = any_w || F.weighted;
+        SampleDev &S = sd[si];
+        std::memset(&S, 0, sizeof(S));
+        S.pa = F.pos.a; S.pb = F.pos.b; S.pl = F.pos.l; S.pw = F.pos.w;
+        S.sa = F.shp.a; S.sb = F.shp.b; S.sl = F.shp.l;
+        S.sax = F.shp.ax; S.say = F.shp.ay; S.se = F.shp.e; S.sw = F.shp.w;
+        S.pstart = F.pos.start;
+        S.L = F.g.L;
+        S.half = F.g.half;
+        S.los = F.g.los;
+        S.K = F.g.jk > 0 ? F.g.jk * F.g.jk * F.g.jk : 0;
+        S.nbins = B;
+        int64_t nb = (int64_t)(1 + S.K) * B;
+        // the kernel accumulates straight into the caller's outputs
+        S.npairs = (unsigned long long *)F.out->npairs;
+        S.sp = F.out->splus;
+        S.sx = F.out->scross;
+        S.dd = F.weighted ? F.out->dd : nullptr;
+        S.var = any_var ? F.out->variance : nullptr;
+        fill<unsigned long long>(S.npairs, 0ull, nb, st);
+        fill<double>(S.sp, 0.0, nb, st);
+        fill<double>(S.sx, 0.0, B, st);
+        if (S.dd) fill<double>(S.dd, 0.0, nb, st);
+        if (S.var) fill<double>(S.var, 0.0, B, st);
+    }
+    SampleDev *sdd = ctx.arena.get<SampleDev>("fast/samples", ns_);
+    MIA_HIP(hipMemcpyAsync(sdd, sd.data(), sizeof(SampleDev) * ns_, hipMemcpyHostToDevice, st));
+    unsigned long long *rechecks = ctx.arena.get<unsigned long long>("fast/rechecks", 1);
+    fill<unsigned long long>(rechecks, 0ull, 1, st);
+
+    // ---- pair kernel
+    PairArgs A{};
+    A.items = items;
+    A.blocks = blocks;
+    A.cost_prefix = cpre;
+    A.n_items = nitems;
+    A.total_cost = total_cost;
+    A.mode = bins.mode;
+    A.n_r = n_r;
+    A.n_2 = n_2;
+    A.weighted = any_w ? 1 : 0;
+    A.var = any_var ? 1 : 0;
+    A.nbins = B;
+    A.T = T;
+    A.X = X;
+    A.lg0 = (float)std::log2(bins.r2_edges[0]);
+    A.lgs = (float)(n_r / std::log2(bins.r2_edges[n_r] / bins.r2_edges[0]));
+    if (bins.mode == MIA_RPPI) {
+        A.x0 = (float)bins.x2_edges[0];
+        A.xs = (float)(n_2 / (bins.x2_edges[n_2] - bins.x2_edges[0]));
+    } else {
+        A.x0 = -1.0f;
+        A.xs = (float)(n_2 / 2.0);
+    }
+    A.samples = sdd;
+    int off = 4 * B;
+    off = (off + 7) & ~7;
+    A.off_sp = off;
+    off += 8 * B;
+    A.off_sx = off;
+    off += 8 * B;
+    A.off_dd = off;
+    if (A.weighted) off += 8 * B;
+    A.off_var = off;
+    if (A.var) off += 8 * B;
+    A.hist_stride = (off + 15) & ~15;
+    A.rechecks = rechecks;
+    size_t thr_bytes = ((size_t)(n_r + 1 + n_2 + 1) * 8 + 15) & ~(size_t)15;
+    size_t lds = thr_bytes + sizeof(ShapeLds) * kMaxBlockShapes * kWavesPerBlock +
+                 (size_t)A.hist_stride * kWavesPerBlock;
+    MIA_REQUIRE(lds <= 160 * 1024, "histogram too large for LDS (n_r*n_2 too big)");
+    int blocks_per_cu = (int)std::max<size_t>(1, std::min<size_t>(8, (160 * 1024) / lds));
+    int waves = tuning && tuning->waves > 0 ? tuning->waves
+                                            : ctx.cu_count * blocks_per_cu * kWavesPerBlock;
+    if (nitems < waves) waves = (int)std::max<int64_t>(1, nitems);
+    A.n_waves = waves;
+    unsigned grid = (unsigned)((waves + kWavesPerBlock - 1) / kWavesPerBlock);
+    if (nitems > 0) {
+        if (bins.mode == MIA_RPPI) {
+            MIA_HIP(hipFuncSetAttribute((const void *)k_pairs<MIA_RPPI>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            hipLaunchKernelGGL(k_pairs<MIA_RPPI>, dim3(grid), dim3(kThreads), lds, st, A);
+        } else {
+            MIA_HIP(hipFuncSetAttribute((const void *)k_pairs<MIA_RMU>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            hipLaunchKernelGGL(k_pairs<MIA_RMU>, dim3(grid), dim3(kThreads), lds, st, A);
+        }
+        MIA_HIP(hipGetLastError());
+    }
+    MIA_HIP(hipEventRecord(ev[2], st));
+    MIA_HIP(hipEventRecord(ev[3], st));
+    MIA_HIP(hipEventSynchronize(ev[3]));
+    float t01 = 0, t12 = 0, t23 = 0;
+    MIA_HIP(hipEventElapsedTime(&t01, ev[0], ev[1]));
+    MIA_HIP(hipEventElapsedTime(&t12, ev[1], ev[2]));
+    MIA_HIP(hipEventElapsedTime(&t23, ev[2], ev[3]));
+    unsigned long long nre = 0;
+    MIA_HIP(hipMemcpy(&nre, rechecks, sizeof(nre), hipMemcpyDeviceToHost));
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    if (stats) {
+        stats->ms_plan = t01;
+        stats->ms_pairs = t12;
+        stats->ms_finalize = t23;
+        stats->candidates = total_cost;
+        stats->items = nitems;
+        stats->rechecks = (int64_t)nre;
+        stats->slabs_per_block = gsz.empty() ? 0 : gsz[0];
+        stats->block_shapes = bsz;
+        stats->waves = waves;
+    }
+}
+
+}  // namespace mia

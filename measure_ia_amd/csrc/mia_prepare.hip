@@ -1,0 +1,335 @@
+// Jackknife region ids and the counting-sort cell list (HBM-bound pass).
+//
+// Every object gets a region id with the reference's strict sub-box test
+// (src/measureia/measure_IA_base.py:404-452) and a cell key: projected-plane
+// column x line-of-sight slab, with cells aligned to the jackknife sub-boxes
+// so that every nominal cell lies inside one region.  Objects the strict test
+// leaves unassigned (exactly on a sub-box face, at 0, or at L) keep region 0,
+// as in the reference, and go to a per-column tail so that every cell range
+// has one region.  A counting sort then scatters the sample into structure-
+// of-arrays order in HBM.
+#include "mia_common.hpp"
+
+#include <cmath>
+
+namespace mia {
+
+namespace {
+
+constexpr int kScanThreads = 256;
+constexpr int kScanPerThread = 8;
+constexpr int kScanTile = kScanThreads * kScanPerThread;
+
+template <class T>
+__global__ void k_fill(T *p, T v, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+// Exclusive scan of one tile per block; writes the tile total to sums.
+template <class T>
+__global__ __launch_bounds__(kScanThreads) void k_scan_tile(const T *in, T *out, int64_t n,
+                                                            T *sums) {
+    __shared__ T part[kScanThreads];
+    int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanPerThread;
+    T v[kScanPerThread];
+    T acc = 0;
+#pragma unroll
+    for (int k = 0; k < kScanPerThread; ++k) {
+        int64_t i = base + k;
+        v[k] = i < n ? in[i] : T(0);
+        acc += v[k];
+    }
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = 1; off < kScanThreads; off <<= 1) {
+        T t = threadIdx.x >= off ? part[threadIdx.x - off] : T(0);
+        __syncthreads();
+        part[threadIdx.x] += t;
+        __syncthreads();
+    }
+    T run = part[threadIdx.x] - acc;  // exclusive prefix of this thread
+#pragma unroll
+    for (int k = 0; k < kScanPerThread; ++k) {
+        int64_t i = base + k;
+        if (i < n) out[i] = run;
+        run += v[k];
+    }
+    if (threadIdx.x == kScanThreads - 1) sums[blockIdx.x] = part[threadIdx.x];
+}
+
+template <class T>
+__global__ void k_scan_add(T *out, int64_t n, const T *offs) {
+    int64_t i = (int64_t)blockIdx.x * kScanTile + threadIdx.x;
+    T o = offs[blockIdx.x];
+    for (int k = 0; k < kScanPerThread; ++k, i += kScanThreads)
+        if (i < n) out[i] += o;
+}
+
+template <class T>
+__global__ void k_set_total(T *out, int64_t n, const T *in, const T *excl) {
+    // out[n] = excl[n-1] + in[n-1]
+    if (threadIdx.x == 0 && blockIdx.x == 0) out[n] = n > 0 ? excl[n - 1] + in[n - 1] : T(0);
+}
+
+__device__ __forceinline__ int axis_cell(double x, int ji, double jk_sub, int per_jk, double width,
+                                         int n) {
+    int k;
+    if (ji >= 0) {
+        double rel = x - (double)ji * jk_sub;
+        double cs = jk_sub / per_jk;
+        double q = rel / cs;
+        q = q < 0.0 ? 0.0 : (q > (double)per_jk ? (double)per_jk : q);
+        k = clampi((int)floor(q), 0, per_jk - 1) + ji * per_jk;
+    } else {
+        double q = x / width;
+        q = q < 0.0 ? 0.0 : (q > (double)n ? (double)n : q);
+        k = clampi((int)floor(q), 0, n - 1);
+    }
+    return k;
+}
+
+// Region id, cell key and per-region statistics of every object.
+__global__ void k_classify(const double *__restrict__ xyz, int64_t n, Grid g,
+                           const double *__restrict__ w, const double *__restrict__ e,
+                           int32_t *__restrict__ key, int32_t *__restrict__ reg,
+                           int32_t *__restrict__ counts, double *__restrict__ rstats, int which,
+                           unsigned long long *__restrict__ wmax, int *__restrict__ oor) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double x[3] = {xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};
+    bool ok = true;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) ok = ok && (x[c] >= 0.0) && (x[c] <= g.L);
+    if (!ok) {
+        atomicOr(oor, 1);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) x[c] = 0.0;
+    }
+    int ji[3] = {-1, -1, -1};
+    bool anom = false;
+    int region = 0;
+    if (g.jk > 0) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) ji[c] = strict_axis_index(x[c], g.jk_sub, g.jk);
+        anom = ji[0] < 0 || ji[1] < 0 || ji[2] < 0;
+        if (!anom) region = ji[0] * g.jk * g.jk + ji[1] * g.jk + ji[2];
+    }
+    int ca = axis_cell(x[g.ax_a], anom ? -1 : ji[g.ax_a], g.jk_sub, g.ncj, g.cw, g.nc);
+    int cb = axis_cell(x[g.ax_b], anom ? -1 : ji[g.ax_b], g.jk_sub, g.ncj, g.cw, g.nc);
+    int cl = axis_cell(x[g.los], anom ? -1 : ji[g.los], g.jk_sub, g.nzj, g.hz, g.nz);
+    int k = (ca * g.nc + cb) * (g.nz + 1) + (anom ? g.nz : cl);
+    key[i] = k;
+    reg[i] = region;
+    atomicAdd(&counts[k], 1);
+    double wi = w ? w[i] : 1.0;
+    if (g.jk > 0 && rstats) {
+        atomicAdd(&rstats[4 * region + which], 1.0);
+        if (which == 1) {
+            double ei = e[i];
+            atomicAdd(&rstats[4 * region + 2], wi);
+            atomicAdd(&rstats[4 * region + 3], wi * (1 - ei * ei / 2.0));
+        }
+    }
+    double aw = fabs(wi);
+    unsigned long long bits = __double_as_longlong(aw);
+    if (aw == aw) atomicMax(wmax, bits);
+}
+
+__global__ void k_scatter(const double *__restrict__ xyz, int64_t n, Grid g,
+                          const double *__restrict__ w, const double *__restrict__ axis,
+                          const double *__restrict__ e, const int32_t *__restrict__ key,
+                          const int32_t *__restrict__ reg, const int32_t *__restrict__ start,
+                          int32_t *__restrict__ cursor, Sorted out) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int k = key[i];
+    int64_t d = (int64_t)start[k] + atomicAdd(&cursor[k], 1);
+    out.a[d] = xyz[3 * i + g.ax_a];
+    out.b[d] = xyz[3 * i + g.ax_b];
+    out.l[d] = xyz[3 * i + g.los];
+    out.w[d] = w ? w[i] : 1.0;
+    out.reg[d] = reg[i];
+    out.key[d] = k;
+    if (axis) {
+        out.ax[d] = axis[2 * i];
+        out.ay[d] = axis[2 * i + 1];
+        out.e[d] = e[i];
+    }
+}
+
+__global__ void k_region_ids(const double *__restrict__ xyz, int64_t n, double jk_sub, int jk,
+                             int32_t *__restrict__ region) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int ji[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) ji[c] = strict_axis_index(xyz[3 * i + c], jk_sub, jk);
+    bool anom = ji[0] < 0 || ji[1] < 0 || ji[2] < 0;
+    region[i] = anom ? 0 : ji[0] * jk * jk + ji[1] * jk + ji[2];
+}
+
+inline unsigned grid_for(int64_t n, int threads) {
+    int64_t b = (n + threads - 1) / threads;
+    return (unsigned)(b < 1 ? 1 : b);
+}
+
+}  // namespace
+
+template <class T>
+void fill(T *p, T v, int64_t n, hipStream_t st) {
+    if (n <= 0) return;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)blocks), dim3(256), 0, st, p, v, n);
+    MIA_HIP(hipGetLastError());
+}
+template void fill<int32_t>(int32_t *, int32_t, int64_t, hipStream_t);
+template void fill<int64_t>(int64_t *, int64_t, int64_t, hipStream_t);
+template void fill<uint64_t>(uint64_t *, uint64_t, int64_t, hipStream_t);
+template void fill<double>(double *, double, int64_t, hipStream_t);
+template void fill<unsigned long long>(unsigned long long *, unsigned long long, int64_t,
+                                       hipStream_t);
+
+// Exclusive scan of in[0..n) into out[0..n], out[n] = total.  Scratch from the arena.
+template <class T>
+void exclusive_scan(Ctx &ctx, const T *in, T *out, int64_t n, hipStream_t st,
+                    const std::string &tag, int depth) {
+    int64_t tiles = (n + kScanTile - 1) / kScanTile;
+    if (tiles < 1) tiles = 1;
+    std::string nm = tag + "/scan" + std::to_string(depth);
+    T *sums = ctx.arena.get<T>(nm + "s", (size_t)tiles + 1);
+    T *soff = ctx.arena.get<T>(nm + "o", (size_t)tiles + 1);
+    hipLaunchKernelGGL(k_scan_tile<T>, dim3((unsigned)tiles), dim3(kScanThreads), 0, st, in, out,
+                       n, sums);
+    MIA_HIP(hipGetLastError());
+    if (tiles > 1) {
+        exclusive_scan<T>(ctx, sums, soff, tiles, st, tag, depth + 1);
+        hipLaunchKernelGGL(k_scan_add<T>, dim3((unsigned)tiles), dim3(kScanThreads), 0, st, out, n,
+                           soff);
+        MIA_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_set_total<T>, dim3(1), dim3(1), 0, st, out, n, in, out);
+    MIA_HIP(hipGetLastError());
+}
+template void exclusive_scan<int32_t>(Ctx &, const int32_t *, int32_t *, int64_t, hipStream_t,
+                                      const std::string &, int);
+template void exclusive_scan<int64_t>(Ctx &, const int64_t *, int64_t *, int64_t, hipStream_t,
+                                      const std::string &, int);
+
+Grid make_grid(const mia_sample &s, const mia_bins &bins, const mia_tuning *t) {
+    Grid g;
+    g.L = s.boxsize;
+    g.half = s.boxsize / 2.0;
+    g.los = s.los;
+    int k = 0;
+    int na[2];
+    for (int c = 0; c < 3; ++c)
+        if (c != s.los) na[k++] = c;
+    g.ax_a = na[0];
+    g.ax_b = na[1];
+    g.jk = s.jk_per_side > 0 ? s.jk_per_side : 0;
+    g.jk_sub = g.jk > 0 ? s.jk_sub : s.boxsize;
+    double reach_r = std::sqrt(bins.r2_edges[bins.n_r]);
+    double reach_z;
+    if (bins.mode == MIA_RPPI)
+        reach_z = std::fmax(std::fabs(bins.x2_edges[0]), std::fabs(bins.x2_edges[bins.n_2]));
+    else
+        reach_z = reach_r;
+    int jk_eff = g.jk > 0 ? g.jk : 1;
+    double span = g.L / jk_eff;
+    // Default: columns ~ reach/4 wide, slabs ~ reach_z/8 high (>= reach/4),
+    // capped so the key space stays small.
+    double cw_t = reach_r / 4.0;
+    double hz_t = std::fmax(reach_z / 8.0, reach_r / 4.0);
+    int ncj = t && t->columns_per_side > 0 ? std::max(1, t->columns_per_side / jk_eff)
+                                           : std::max(1, (int)std::lround(span / cw_t));
+    int nzj = t && t->slabs_per_side > 0 ? std::max(1, t->slabs_per_side / jk_eff)
+                                         : std::max(1, (int)std::lround(span / hz_t));
+    while (ncj * jk_eff > 512 && ncj > 1) --ncj;
+    while (nzj * jk_eff > 512 && nzj > 1) --nzj;
+    g.ncj = ncj;
+    g.nzj = nzj;
+    g.nc = ncj * jk_eff;
+    g.nz = nzj * jk_eff;
+    g.cw = g.L / g.nc;
+    g.hz = g.L / g.nz;
+    g.margin = 1e-9 * g.L + 1e-12;
+    return g;
+}
+
+void region_ids(Ctx &ctx, const double *xyz, int64_t n, double jk_sub, int jk, int32_t *region,
+                hipStream_t st) {
+    (void)ctx;
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_region_ids, dim3(grid_for(n, 256)), dim3(256), 0, st, xyz, n, jk_sub, jk,
+                       region);
+    MIA_HIP(hipGetLastError());
+}
+
+static void sort_one(Ctx &ctx, const double *xyz, int64_t n, const double *w, const double *axis,
+                     const double *e, const Grid &g, Sorted &out, double *rstats, int which,
+                     unsigned long long *wmax, int *oor, hipStream_t st, const std::string &tag) {
+    int keys = g.keys();
+    out.n = n;
+    size_t nn = (size_t)(n > 0 ? n : 1);
+    out.a = ctx.arena.get<double>(tag + "/a", nn);
+    out.b = ctx.arena.get<double>(tag + "/b", nn);
+    out.l = ctx.arena.get<double>(tag + "/l", nn);
+    out.w = ctx.arena.get<double>(tag + "/w", nn);
+    out.reg = ctx.arena.get<int32_t>(tag + "/reg", nn);
+    out.key = ctx.arena.get<int32_t>(tag + "/key", nn);
+    if (axis) {
+        out.ax = ctx.arena.get<double>(tag + "/ax", nn);
+        out.ay = ctx.arena.get<double>(tag + "/ay", nn);
+        out.e = ctx.arena.get<double>(tag + "/e", nn);
+    }
+    out.start = ctx.arena.get<int32_t>(tag + "/start", (size_t)keys + 1);
+    int32_t *counts = ctx.arena.get<int32_t>(tag + "/counts", (size_t)keys + 1);
+    int32_t *cursor = ctx.arena.get<int32_t>(tag + "/cursor", (size_t)keys + 1);
+    int32_t *key0 = ctx.arena.get<int32_t>(tag + "/key0", nn);
+    int32_t *reg0 = ctx.arena.get<int32_t>(tag + "/reg0", nn);
+    fill<int32_t>(counts, 0, keys + 1, st);
+    fill<int32_t>(cursor, 0, keys + 1, st);
+    if (n > 0) {
+        hipLaunchKernelGGL(k_classify, dim3(grid_for(n, 256)), dim3(256), 0, st, xyz, n, g, w, e,
+                           key0, reg0, counts, rstats, which, wmax, oor);
+        MIA_HIP(hipGetLastError());
+    }
+    exclusive_scan<int32_t>(ctx, counts, out.start, keys, st, tag, 0);
+    if (n > 0) {
+        hipLaunchKernelGGL(k_scatter, dim3(grid_for(n, 256)), dim3(256), 0, st, xyz, n, g, w, axis,
+                           e, key0, reg0, out.start, cursor, out);
+        MIA_HIP(hipGetLastError());
+    }
+}
+
+void prepare_sample(Ctx &ctx, const mia_sample &s, Grid &g, Sorted &pos, Sorted &shp,
+                    double *region_stats, Sums &sums, hipStream_t st, const std::string &tag) {
+    unsigned long long *wmax = ctx.arena.get<unsigned long long>(tag + "/wmax", 2);
+    int *oor = ctx.arena.get<int>(tag + "/oor", 1);
+    fill<unsigned long long>(wmax, 0ull, 2, st);
+    fill<int32_t>(oor, 0, 1, st);
+    if (region_stats && g.jk > 0) fill<double>(region_stats, 0.0, (int64_t)g.jk * g.jk * g.jk * 4, st);
+    double *rs = g.jk > 0 ? region_stats : nullptr;
+    sort_one(ctx, s.pos, s.n_pos, s.pos_w, nullptr, nullptr, g, pos, rs, 0, wmax, oor, st,
+             tag + "/pos");
+    sort_one(ctx, s.shp, s.n_shape, s.shp_w, s.axis, s.e, g, shp, rs, 1, wmax + 1, oor, st,
+             tag + "/shp");
+    unsigned long long hw[2];
+    int hoor = 0;
+    MIA_HIP(hipMemcpyAsync(hw, wmax, sizeof(hw), hipMemcpyDeviceToHost, st));
+    MIA_HIP(hipMemcpyAsync(&hoor, oor, sizeof(int), hipMemcpyDeviceToHost, st));
+    MIA_HIP(hipStreamSynchronize(st));
+    double dw[2];
+    std::memcpy(dw, hw, sizeof(dw));
+    sums.wmax_pos = dw[0];
+    sums.wmax_shape = dw[1];
+    sums.out_of_range = hoor;
+    if (hoor)
+        throw Error{MIA_E_RANGE,
+                    "coordinates outside [0, boxsize] (or NaN) in a periodic sample"};
+}
+
+}  // namespace mia

@@ -1,0 +1,227 @@
+"""Device driver of the pair-counting engine (torch plumbing + the C ABI).
+
+torch owns the device memory of inputs and outputs and supplies the stream;
+all counting happens in libmeasure_ia_amd.so.  There is no CPU fallback: if
+the library or a GPU is missing, constructing an Engine raises.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _capi
+from .binning import Thresholds
+
+METHODS = {"plain": _capi.MIA_PLAIN, "fast": _capi.MIA_FAST}
+
+
+@dataclass
+class SampleArrays:
+    """One galaxy sample as the box counters prepare it (host or device arrays)."""
+    positions: object            # (N_p, 3) float64
+    shapes: object               # (N_s, 3) float64
+    axis: object                 # (N_s, 2) float64, normalised
+    e: object                    # (N_s,)   float64
+    weight: Optional[object] = None        # (N_p,) or None (unit)
+    weight_shape: Optional[object] = None  # (N_s,) or None (unit)
+    boxsize: float = 0.0
+    los: int = 2
+    jk_per_side: int = 0         # L_subboxes (0: no jackknife)
+    jk_sub: float = 0.0          # L_sub = 2*L_0p5/L_subboxes
+    periodic: bool = True
+
+
+@dataclass
+class CountResult:
+    npairs: np.ndarray           # (1+K, n_r, n_2) int64 -- [0] total, [1+i] tally of region i
+    dd: np.ndarray               # (1+K, n_r, n_2) float64
+    splus: np.ndarray            # (1+K, n_r, n_2) float64: sum w_p w_s e+ (no 1/2R)
+    scross: np.ndarray           # (n_r, n_2) float64
+    variance: Optional[np.ndarray]
+    region_stats: Optional[np.ndarray]   # (K, 4): n_pos, n_shape, sum w_s, sum w_s(1-e^2/2)
+    stats: dict = field(default_factory=dict)
+
+
+def _unit(w) -> bool:
+    if w is None:
+        return True
+    try:
+        import torch
+        if isinstance(w, torch.Tensor):
+            return bool(torch.all(w == 1).item())
+    except ImportError:
+        pass
+    return bool(np.all(np.asarray(w) == 1.0))
+
+
+class Engine:
+    """A context of the HIP library on one device."""
+
+    def __init__(self, device: int = 0):
+        import torch
+        if not torch.cuda.is_available():
+            raise RuntimeError("measure_ia_amd: no HIP device visible (the engine has no CPU path)")
+        self.torch = torch
+        self.device = device
+        self.dev = torch.device("cuda", device)
+        self.lib = _capi.load()
+        ctx = ctypes.c_void_p()
+        rc = self.lib.mia_create(device, ctypes.byref(ctx))
+        if rc != 0:
+            raise _capi.MiaError(rc, (self.lib.mia_last_error(None) or b"").decode())
+        self.ctx = ctx
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.lib.mia_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- helpers -------------------------------------------------------------
+    def _dev(self, a, shape=None):
+        t = self.torch
+        if a is None:
+            return None
+        if isinstance(a, t.Tensor):
+            x = a.to(device=self.dev, dtype=t.float64).contiguous()
+        else:
+            x = t.from_numpy(np.ascontiguousarray(np.asarray(a, dtype=np.float64))).to(self.dev)
+        if shape is not None and tuple(x.shape) != shape:
+            raise ValueError(f"array shape {tuple(x.shape)} != {shape}")
+        return x
+
+    @staticmethod
+    def _bins(th: Thresholds, variance: bool):
+        r2 = np.ascontiguousarray(th.r2_edges, dtype=np.float64)
+        x2 = np.ascontiguousarray(th.x2_edges, dtype=np.float64)
+        b = _capi.MiaBins(mode=_capi.MIA_RPPI if th.mode == "rppi" else _capi.MIA_RMU,
+                          n_r=th.n_r, n_2=th.n_2, accumulate_variance=1 if variance else 0,
+                          r2_edges=r2.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                          x2_edges=x2.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return b, (r2, x2)
+
+    @staticmethod
+    def _tuning(tuning):
+        if not tuning:
+            return None
+        t = _capi.MiaTuning()
+        for k, v in tuning.items():
+            setattr(t, k, int(v))
+        return t
+
+    def _prepare(self, s: SampleArrays):
+        n_p = int(s.positions.shape[0])
+        n_s = int(s.shapes.shape[0])
+        pos = self._dev(s.positions, (n_p, 3))
+        shp = self._dev(s.shapes, (n_s, 3))
+        axis = self._dev(s.axis, (n_s, 2))
+        e = self._dev(s.e, (n_s,))
+        wp = None if _unit(s.weight) else self._dev(s.weight, (n_p,))
+        ws = None if _unit(s.weight_shape) else self._dev(s.weight_shape, (n_s,))
+        keep = [pos, shp, axis, e, wp, ws]
+
+        def ptr(x):
+            return None if x is None else x.data_ptr()
+
+        ms = _capi.MiaSample(pos=ptr(pos), pos_w=ptr(wp), shp=ptr(shp), shp_w=ptr(ws),
+                             axis=ptr(axis), e=ptr(e), n_pos=n_p, n_shape=n_s,
+                             boxsize=float(s.boxsize), jk_sub=float(s.jk_sub), los=int(s.los),
+                             periodic=1 if s.periodic else 0, jk_per_side=int(s.jk_per_side))
+        return ms, keep, (wp is not None or ws is not None)
+
+    def _outputs(self, K, nb, variance):
+        t = self.torch
+        o = dict(npairs=t.zeros((1 + K) * nb, dtype=t.int64, device=self.dev),
+                 dd=t.zeros((1 + K) * nb, dtype=t.float64, device=self.dev),
+                 splus=t.zeros((1 + K) * nb, dtype=t.float64, device=self.dev),
+                 scross=t.zeros(nb, dtype=t.float64, device=self.dev),
+                 variance=t.zeros(nb, dtype=t.float64, device=self.dev) if variance else None,
+                 region_stats=t.zeros(max(K, 1) * 4, dtype=t.float64, device=self.dev) if K else None)
+        c = _capi.MiaCounts(**{k: (None if v is None else v.data_ptr()) for k, v in o.items()})
+        return o, c
+
+    @staticmethod
+    def _host(o, K, th: Thresholds, stats):
+        shp3 = (1 + K, th.n_r, th.n_2)
+        return CountResult(
+            npairs=o["npairs"].cpu().numpy().reshape(shp3),
+            dd=o["dd"].cpu().numpy().reshape(shp3),
+            splus=o["splus"].cpu().numpy().reshape(shp3),
+            scross=o["scross"].cpu().numpy().reshape(th.n_r, th.n_2),
+            variance=None if o["variance"] is None else o["variance"].cpu().numpy().reshape(th.n_r, th.n_2),
+            region_stats=None if o["region_stats"] is None else o["region_stats"].cpu().numpy().reshape(K, 4),
+            stats=stats)
+
+    # -- API -----------------------------------------------------------------
+    def count(self, sample: SampleArrays, th: Thresholds, method: str = "fast",
+              variance: bool = False, tuning: Optional[dict] = None) -> CountResult:
+        """Count one sample; returns host arrays."""
+        ms, keep, _ = self._prepare(sample)
+        K = sample.jk_per_side ** 3 if sample.jk_per_side > 0 else 0
+        nb = th.n_r * th.n_2
+        bins, keep_b = self._bins(th, variance)
+        o, c = self._outputs(K, nb, variance)
+        st = _capi.MiaStats()
+        tu = self._tuning(tuning)
+        stream = self.torch.cuda.current_stream(self.dev).cuda_stream
+        rc = self.lib.mia_count(self.ctx, ctypes.byref(ms), ctypes.byref(bins), ctypes.byref(c),
+                                METHODS[method], ctypes.byref(tu) if tu else None,
+                                ctypes.byref(st), ctypes.c_void_p(stream))
+        _capi.check(self.lib, self.ctx, rc)
+        return self._host(o, K, th, st.as_dict())
+
+    def count_batch(self, samples: Sequence[SampleArrays], th: Thresholds,
+                    variance: bool = False, tuning: Optional[dict] = None):
+        """Count many samples in one persistent launch; returns a list of results."""
+        n = len(samples)
+        mss = (_capi.MiaSample * n)()
+        cs = (_capi.MiaCounts * n)()
+        keep, outs, Ks = [], [], []
+        nb = th.n_r * th.n_2
+        for i, s in enumerate(samples):
+            ms, k, _ = self._prepare(s)
+            keep.append(k)
+            mss[i] = ms
+            K = s.jk_per_side ** 3 if s.jk_per_side > 0 else 0
+            o, c = self._outputs(K, nb, variance)
+            outs.append(o)
+            Ks.append(K)
+            cs[i] = c
+        bins, keep_b = self._bins(th, variance)
+        st = _capi.MiaStats()
+        tu = self._tuning(tuning)
+        stream = self.torch.cuda.current_stream(self.dev).cuda_stream
+        rc = self.lib.mia_count_batch(self.ctx, n, mss, ctypes.byref(bins), cs,
+                                      ctypes.byref(tu) if tu else None, ctypes.byref(st),
+                                      ctypes.c_void_p(stream))
+        _capi.check(self.lib, self.ctx, rc)
+        stats = st.as_dict()
+        return [self._host(o, K, th, stats) for o, K in zip(outs, Ks)]
+
+    def region_ids(self, xyz, jk_per_side: int, jk_sub: float) -> np.ndarray:
+        x = self._dev(xyz)
+        n = int(x.shape[0])
+        out = self.torch.zeros(max(n, 1), dtype=self.torch.int32, device=self.dev)
+        stream = self.torch.cuda.current_stream(self.dev).cuda_stream
+        rc = self.lib.mia_region_ids(self.ctx, x.data_ptr(), n, float(jk_sub), int(jk_per_side),
+                                     out.data_ptr(), ctypes.c_void_p(stream))
+        _capi.check(self.lib, self.ctx, rc)
+        return out[:n].cpu().numpy()
+
+
+_ENGINES = {}
+
+
+def get_engine(device: int = 0) -> Engine:
+    e = _ENGINES.get(device)
+    if e is None:
+        e = _ENGINES[device] = Engine(device)
+    return e
