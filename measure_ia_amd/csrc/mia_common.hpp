@@ -95,6 +95,7 @@ struct Sorted {
     int32_t *reg = nullptr;                           // jackknife region (0 without jk)
     int32_t *key = nullptr;                           // cell key
     double *ax = nullptr, *ay = nullptr, *e = nullptr; // shapes only
+    double *rec = nullptr;  // shapes only: 8 doubles per shape {a, b, l, ax, ay, e*w, w, e}
     int32_t *start = nullptr;                         // keys+1 offsets
 };
 

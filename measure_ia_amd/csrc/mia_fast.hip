@@ -54,11 +54,13 @@ struct Item {
     uint8_t pad;
 };
 static_assert(sizeof(Item) == 16, "Item layout");
+static_assert(sizeof(Block) == 24, "Block layout");
 
 struct SampleDev {
     // sorted sample
     const double *pa, *pb, *pl, *pw;
     const double *sa, *sb, *sl, *sax, *say, *se, *sw;
+    const double *srec;          // 8 doubles per shape, sorted order
     const int32_t *pstart;
     double L, half;
     int los, K, nbins, pad;
@@ -289,20 +291,22 @@ struct PairArgs {
     int64_t total_cost;
     int n_waves;
     int mode, n_r, n_2, weighted, var, nbins;
-    const double *T, *X;
-    float lg0, lgs;              // log2(T[0]), n_r / log2(T[n_r]/T[0])
-    float x0, xs;                // pi: X[0], n_2/(X[n_2]-X[0]);  mu: -1, n_2/2
+    const double *T, *X;         // exact thresholds (global copy)
+    double T0, Tn, X0, Xn;       // domain bounds (rppi: s and pi; rmu: r^2)
+    float lg0, lgs;              // bin guess: (log2(v) - lg0) * lgs
+    float x0, xs;                // pi: (pi - x0) * xs; mu: (mu + 1) * xs
+    float dr, d2;                // FP32 guard bands in bin units
+    int exact_only;              // 1: skip the FP32 screen (pathological binning)
     const SampleDev *samples;
     int hist_stride;             // bytes per wave histogram
+    int copies_log2;             // each bin is kept in 2^copies_log2 interleaved copies
     int off_sp, off_sx, off_dd, off_var;  // byte offsets inside a wave histogram
     unsigned long long *rechecks;
 };
 
-struct ShapeLds {
-    double a, b, l;
-    double ax, ay, ew, w;
-    double pad;
-};
+// Shape record {a, b, l, ax, ay, e*w, w, e} read through the scalar cache: the
+// inner loop is uniform over shapes, so every field lands in SGPRs.
+typedef const __attribute__((address_space(4))) double *ShapePtr;
 
 __device__ void flush_hist(const PairArgs &P, char *hist, const SampleDev &S, int rs, int rp,
                            int lane) {
@@ -312,17 +316,28 @@ __device__ void flush_hist(const PairArgs &P, char *hist, const SampleDev &S, in
     double *dd = P.weighted ? (double *)(hist + P.off_dd) : nullptr;
     double *vr = P.var ? (double *)(hist + P.off_var) : nullptr;
     const int B = P.nbins;
+    const int C = 1 << P.copies_log2;
     for (int b = lane; b < B; b += kWave) {
-        uint32_t c = cnt[b];
+        uint32_t c = 0;
+        double vsp = 0.0, vsx = 0.0, vdd = 0.0, vvr = 0.0;
+        for (int k = 0; k < C; ++k) {
+            int h = b * C + k;
+            c += cnt[h];
+            vsp += sp[h];
+            vsx += sx[h];
+            cnt[h] = 0;
+            sp[h] = 0.0;
+            sx[h] = 0.0;
+            if (dd) {
+                vdd += dd[h];
+                dd[h] = 0.0;
+            }
+            if (vr) {
+                vvr += vr[h];
+                vr[h] = 0.0;
+            }
+        }
         if (c == 0) continue;
-        double vsp = sp[b], vsx = sx[b];
-        double vdd = dd ? dd[b] : 0.0;
-        double vvr = vr ? vr[b] : 0.0;
-        cnt[b] = 0;
-        sp[b] = 0.0;
-        sx[b] = 0.0;
-        if (dd) dd[b] = 0.0;
-        if (vr) vr[b] = 0.0;
         atomicAdd(&S.npairs[b], (unsigned long long)c);
         atomicAdd(&S.sp[b], vsp);
         atomicAdd(&S.sx[b], vsx);
@@ -353,9 +368,41 @@ __device__ __forceinline__ double r2_column_order(double a2, double b2, double l
     return (l2 + a2) + b2;
 }
 
+// #{k in 1..n-1 : E[k] <= v}, the exact bin of v in [E[0], E[n]).
+__device__ __forceinline__ int count_edges(const double *E, int n, double v) {
+    int k = 0;
+    for (int i = 1; i < n; ++i) k += (E[i] <= v);
+    return k;
+}
+
+#define MIA_CONST __attribute__((address_space(4)))
+
+// Per-sample values the pair loop needs, read once per item through the
+// scalar cache (they are wave-uniform and end up in SGPRs).
+struct SampleView {
+    const double *pa, *pb, *pl, *pw;
+    const double *srec;
+    double L, half;
+    int los;
+};
+
+__device__ __forceinline__ SampleView load_view(const SampleDev *sd) {
+    const MIA_CONST SampleDev *c = (const MIA_CONST SampleDev *)sd;
+    SampleView v;
+    v.pa = c->pa;
+    v.pb = c->pb;
+    v.pl = c->pl;
+    v.pw = c->pw;
+    v.srec = c->srec;
+    v.L = c->L;
+    v.half = c->half;
+    v.los = c->los;
+    return v;
+}
+
 template <int MODE, bool WRAP>
-__device__ __forceinline__ void run_item(const PairArgs &P, const SampleDev &S, const Item &it,
-                                         const ShapeLds *sh, int ns, int s0, char *hist,
+__device__ __forceinline__ void run_item(const PairArgs &P, const SampleView &S, int p0, int np,
+                                         int shifts, ShapePtr shp, int ns, char *hist,
                                          const double *T, const double *X, int lane) {
     uint32_t *cnt = (uint32_t *)hist;
     double *hsp = (double *)(hist + P.off_sp);
@@ -363,21 +410,30 @@ __device__ __forceinline__ void run_item(const PairArgs &P, const SampleDev &S, 
     double *hdd = (double *)(hist + P.off_dd);
     double *hvr = (double *)(hist + P.off_var);
     const double L = S.L, half = S.half;
-    const double sha = WRAP ? 0.0 : shift_val(it.shifts & 3, L);
-    const double shb = WRAP ? 0.0 : shift_val((it.shifts >> 2) & 3, L);
-    const double shl = WRAP ? 0.0 : shift_val((it.shifts >> 4) & 3, L);
-    const double T0 = T[0], Tn = T[P.n_r];
+    const double sha = WRAP ? 0.0 : shift_val(shifts & 3, L);
+    const double shb = WRAP ? 0.0 : shift_val((shifts >> 2) & 3, L);
+    const double shl = WRAP ? 0.0 : shift_val((shifts >> 4) & 3, L);
     const int n_r = P.n_r, n_2 = P.n_2;
-    for (int base = 0; base < it.np; base += kWave) {
-        int pi = base + lane;
-        bool valid = pi < it.np;
-        int p = it.p0 + (valid ? pi : 0);
+    const int copy = lane & ((1 << P.copies_log2) - 1);
+    // Strided lane -> position map: lane l of batch b takes position b + l*nb,
+    // so one batch spans the whole line-of-sight range of the item and its
+    // pairs spread over more (r, pi) bins (fewer same-address LDS atomics).
+    const int nb = (np + kWave - 1) / kWave;
+    for (int bt = 0; bt < nb; ++bt) {
+        int pi = bt + lane * nb;
+        bool valid = pi < np;
+        int p = p0 + (valid ? pi : 0);
         double pa = S.pa[p], pb = S.pb[p], pl = S.pl[p];
         double pw = S.pw[p];
+        double qa = shp[0], qb = shp[1], ql = shp[2], qax = shp[3], qay = shp[4], qew = shp[5],
+               qw = shp[6];
         for (int j = 0; j < ns; ++j) {
-            const ShapeLds q = sh[j];
+            // prefetch the next shape record (scalar loads) while this one is used
+            ShapePtr rn = shp + 8 * (j + 1 < ns ? j + 1 : j);
+            const double na = rn[0], nbv = rn[1], nl = rn[2], nax = rn[3], nay = rn[4],
+                         new_ = rn[5], nw = rn[6];
             // the reference's own FP64 separation (measure_w_box_jk.py:159-162)
-            double da = q.a - pa, db = q.b - pb, dl = q.l - pl;
+            double da = qa - pa, db = qb - pb, dl = ql - pl;
             if (WRAP) {
                 da = wrap_sep(da, L, half);
                 db = wrap_sep(db, L, half);
@@ -393,78 +449,75 @@ __device__ __forceinline__ void run_item(const PairArgs &P, const SampleDev &S, 
             bool in;
             if (MODE == MIA_RPPI) {
                 rv = s;
-                in = valid && s >= T0 && s < Tn && dl >= X[0] && dl < X[n_2];
+                in = valid & (s >= P.T0) & (s < P.Tn) & (dl >= P.X0) & (dl < P.Xn);
             } else {
                 rv = r2_column_order(a2, b2, dl * dl, S.los);
-                in = valid && s > 0.0 && rv >= T0 && rv < Tn;
+                in = valid & (s > 0.0) & (rv >= P.T0) & (rv < P.Tn);
             }
-            if (!__any(in)) continue;
-            if (!in) continue;
-            // r bin: FP32 log2 guess, exact FP64 correction against the thresholds
-            float rv32 = (float)rv;
-            float gr = (__log2f(rv32) - P.lg0) * P.lgs;
-            gr = fminf(fmaxf(gr, 0.0f), (float)(n_r - 1));
-            int ir = (int)gr;
-            while (ir < n_r - 1 && rv >= T[ir + 1]) ++ir;
-            while (ir > 0 && rv < T[ir]) --ir;
-            int i2;
-            float dlf = (float)dl;
-            if (MODE == MIA_RPPI) {
-                float g2 = (dlf - P.x0) * P.xs;
-                g2 = fminf(fmaxf(g2, 0.0f), (float)(n_2 - 1));
-                i2 = (int)g2;
-                while (i2 < n_2 - 1 && dl >= X[i2 + 1]) ++i2;
-                while (i2 > 0 && dl < X[i2]) --i2;
-            } else {
-                // mu = pi / r screened in FP32; exact FP64 pi / sqrt(r^2) near an edge
-                float mu = dlf * __builtin_amdgcn_rsqf(rv32);
-                float g2 = (mu + 1.0f) * P.xs;
-                g2 = fminf(fmaxf(g2, 0.0f), (float)(n_2 - 1));
-                i2 = (int)g2;
-                float lo = (float)X[i2];
-                float hi = i2 + 1 < n_2 ? (float)X[i2 + 1] : 3.0f;
-                bool sure = (mu - kMuBand >= lo) && (mu + kMuBand < hi);
-                if (!sure) {
-                    double mu64 = dl / sqrt(rv);
-                    if (mu64 < X[0]) {
-                        i2 = n_2 - 1;  // numpy's index -1
+            if (__any(in) && in) {
+                // bins: FP32 guess; exact FP64 threshold compare only within the
+                // guard band of an interior edge (the domain edges are exact above)
+                float rv32 = (float)rv;
+                float dlf = (float)dl;
+                float gr = (__log2f(rv32) - P.lg0) * P.lgs;
+                float g2;
+                if (MODE == MIA_RPPI)
+                    g2 = (dlf - P.x0) * P.xs;
+                else
+                    g2 = __fmaf_rn(dlf, __builtin_amdgcn_rsqf(rv32), 1.0f) * P.xs;
+                int ir = clampi((int)floorf(gr), 0, n_r - 1);
+                int i2 = clampi((int)floorf(g2), 0, n_2 - 1);
+                float fr = gr - (float)ir, f2 = g2 - (float)i2;
+                bool near = ((ir > 0) & (fr < P.dr)) | ((ir < n_r - 1) & (fr > 1.0f - P.dr));
+                if (MODE == MIA_RPPI)
+                    near = near | ((i2 > 0) & (f2 < P.d2)) | ((i2 < n_2 - 1) & (f2 > 1.0f - P.d2));
+                else  // bin 0's lower edge M_0 is a real edge for mu
+                    near = near | (f2 < P.d2) | ((i2 < n_2 - 1) & (f2 > 1.0f - P.d2));
+                if (near | (P.exact_only != 0)) {
+                    ir = count_edges(T, n_r, rv);
+                    if (MODE == MIA_RPPI) {
+                        i2 = count_edges(X, n_2, dl);
                     } else {
-                        while (i2 < n_2 - 1 && mu64 >= X[i2 + 1]) ++i2;
-                        while (i2 > 0 && mu64 < X[i2]) --i2;
+                        double mu64 = dl / sqrt(rv);
+                        i2 = mu64 < X[0] ? n_2 - 1 : count_edges(X, n_2, mu64);
                     }
                     atomicAdd(P.rechecks, 1ull);
                 }
-            }
-            // e+ = e cos 2phi = e (2 cos^2 phi - 1), ex = e sin 2phi = 2 e cos phi |sin phi|
-            // (|.| because phi = arccos(d) lies in [0, pi]); cos phi = t/r_p,
-            // |sin phi| = |c|/r_p with t, c the dot and cross products.
-            double t = fma(da, q.ax, db * q.ay);
-            double c = fma(da, q.ay, -(db * q.ax));
-            double r0 = __builtin_amdgcn_rcp(s);
-            double inv = fma(r0, fma(-s, r0, 1.0), r0);
-            double cos2 = t * t * inv;
-            double ep = q.ew * fma(2.0, cos2, -1.0);
-            double ex = 2.0 * q.ew * t * fabs(c) * inv;
-            if (c * c * inv < kSin2Exact) {
-                // nearly parallel: the reference's d may round outside [-1, 1],
-                // where arccos gives NaN and the pair adds no e+ (measure_w_box_jk.py:170-175)
-                int sidx = s0 + j;
-                double rp = sqrt(s);
-                double d = 0.0 + (da / rp) * S.sax[sidx];
-                d = d + (db / rp) * S.say[sidx];
-                if (!(fabs(d) <= 1.0)) {
-                    ep = 0.0;
-                    ex = 0.0;
+                // e+ = e cos 2phi = e (2 cos^2 phi - 1), ex = e sin 2phi = 2 e cos phi |sin phi|
+                // (|.| because phi = arccos(d) lies in [0, pi]); cos phi = t/r_p,
+                // |sin phi| = |c|/r_p with t, c the dot and cross products.
+                double t = fma(da, qax, db * qay);
+                double c = fma(da, qay, -(db * qax));
+                double r0 = __builtin_amdgcn_rcp(s);
+                double inv = fma(r0, fma(-s, r0, 1.0), r0);
+                double ep = qew * fma(2.0 * inv, t * t, -1.0);
+                double ex = 2.0 * qew * t * fabs(c) * inv;
+                if (c * c * inv < kSin2Exact) {
+                    // nearly parallel: the reference's d may round outside [-1, 1],
+                    // where arccos gives NaN and the pair adds no e+ (measure_w_box_jk.py:170-175)
+                    double rp = sqrt(s);
+                    double d = 0.0 + (da / rp) * qax;
+                    d = d + (db / rp) * qay;
+                    if (!(fabs(d) <= 1.0)) {
+                        ep = 0.0;
+                        ex = 0.0;
+                    }
                 }
-                atomicAdd(P.rechecks, 1ull);
+                int b = ((ir * n_2 + i2) << P.copies_log2) + copy;
+                double vsp = pw * ep;
+                atomicAdd(&cnt[b], 1u);
+                atomicAdd(&hsp[b], vsp);
+                atomicAdd(&hsx[b], pw * ex);
+                if (P.weighted) atomicAdd(&hdd[b], pw * qw);
+                if (P.var) atomicAdd(&hvr[b], vsp * vsp);
             }
-            int b = ir * n_2 + i2;
-            double vsp = pw * ep;
-            atomicAdd(&cnt[b], 1u);
-            atomicAdd(&hsp[b], vsp);
-            atomicAdd(&hsx[b], pw * ex);
-            if (P.weighted) atomicAdd(&hdd[b], pw * q.w);
-            if (P.var) atomicAdd(&hvr[b], vsp * vsp);
+            qa = na;
+            qb = nbv;
+            ql = nl;
+            qax = nax;
+            qay = nay;
+            qew = new_;
+            qw = nw;
         }
     }
 }
@@ -473,22 +526,21 @@ template <int MODE>
 __global__ __launch_bounds__(kThreads) void k_pairs(PairArgs P) {
     extern __shared__ __align__(16) char lds[];
     const int lane = threadIdx.x & (kWave - 1);
-    const int wv = threadIdx.x / kWave;
-    // thresholds (shared by the block's waves)
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    // exact thresholds for the rare FP64 bin path (shared by the block's waves)
     double *T = (double *)lds;
     double *X = T + (P.n_r + 1);
     int nx = P.n_2 + 1;
     for (int i = threadIdx.x; i < P.n_r + 1; i += blockDim.x) T[i] = P.T[i];
     for (int i = threadIdx.x; i < nx; i += blockDim.x) X[i] = P.X[i];
     size_t thr_bytes = ((size_t)(P.n_r + 1 + nx) * 8 + 15) & ~(size_t)15;
-    ShapeLds *sh = (ShapeLds *)(lds + thr_bytes) + wv * kMaxBlockShapes;
-    char *hist = lds + thr_bytes + sizeof(ShapeLds) * kMaxBlockShapes * kWavesPerBlock +
-                 (size_t)wv * P.hist_stride;
+    char *hist = lds + thr_bytes + (size_t)wv * P.hist_stride;
     for (int i = lane * 4; i < P.hist_stride; i += kWave * 4) *(uint32_t *)(hist + i) = 0u;
     __syncthreads();
 
-    int w = blockIdx.x * kWavesPerBlock + wv;
+    const int w = blockIdx.x * kWavesPerBlock + wv;
     if (w >= P.n_waves || P.n_items == 0) return;
+    const MIA_CONST int64_t *cpre = (const MIA_CONST int64_t *)P.cost_prefix;
     // items [it0, it1) whose cost prefix falls in this wave's share
     auto first_at_least = [&](int64_t target) {
         int64_t lo = 0, hi = P.n_items;
@@ -496,55 +548,44 @@ __global__ __launch_bounds__(kThreads) void k_pairs(PairArgs P) {
             int64_t mid = (lo + hi) >> 1;
             // prefix[mid] * W >= target * C (any monotone rule works: the
             // neighbouring wave evaluates the same boundary)
-            if ((double)P.cost_prefix[mid] * P.n_waves >= (double)target * (double)P.total_cost)
+            if ((double)cpre[mid] * P.n_waves >= (double)target * (double)P.total_cost)
                 hi = mid;
             else
                 lo = mid + 1;
         }
         return lo;
     };
-    int64_t it0 = first_at_least(w);
-    int64_t it1 = first_at_least(w + 1);
+    const int64_t it0 = first_at_least(w);
+    const int64_t it1 = first_at_least(w + 1);
+    const MIA_CONST int32_t *items = (const MIA_CONST int32_t *)P.items;
+    const MIA_CONST int32_t *blocks = (const MIA_CONST int32_t *)P.blocks;
     int cur_key_s = -1, cur_rs = -1, cur_rp = -1;
-    int cur_blk = -1;
     const SampleDev *Sp = nullptr;
-    int ns = 0, s0 = 0;
+    SampleView S{};
     for (int64_t i = it0; i < it1; ++i) {
-        Item it = P.items[i];
-        Block B = P.blocks[it.blk];
-        if (B.sample != cur_key_s || B.rs != cur_rs || it.rp != cur_rp) {
+        const MIA_CONST int32_t *ip = items + 4 * i;
+        const int blk = ip[0], p0 = ip[1], np = ip[2], w3 = ip[3];
+        const int rp = (int16_t)(w3 & 0xffff);
+        const int shifts = (w3 >> 16) & 0xff;
+        const MIA_CONST int32_t *bp = blocks + 6 * (int64_t)blk;
+        const int s0 = bp[0], ns = bp[1], sample = bp[4], rs = bp[5];
+        if (sample != cur_key_s || rs != cur_rs || rp != cur_rp) {
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             if (Sp) flush_hist(P, hist, *Sp, cur_rs, cur_rp, lane);
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            cur_key_s = B.sample;
-            cur_rs = B.rs;
-            cur_rp = it.rp;
-            Sp = &P.samples[B.sample];
-        }
-        const SampleDev &S = *Sp;
-        if (it.blk != cur_blk) {
-            cur_blk = it.blk;
-            ns = B.ns;
-            s0 = B.s0;
-            if (lane < ns) {
-                int si = s0 + lane;
-                ShapeLds q;
-                q.a = S.sa[si];
-                q.b = S.sb[si];
-                q.l = S.sl[si];
-                q.ax = S.sax[si];
-                q.ay = S.say[si];
-                q.w = S.sw[si];
-                q.ew = S.se[si] * q.w;
-                q.pad = 0.0;
-                sh[lane] = q;
+            if (sample != cur_key_s) {
+                Sp = &P.samples[sample];
+                S = load_view(Sp);
             }
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            cur_key_s = sample;
+            cur_rs = rs;
+            cur_rp = rp;
         }
-        if (it.shifts & 64)
-            run_item<MODE, true>(P, S, it, sh, ns, s0, hist, T, X, lane);
+        ShapePtr shp = (ShapePtr)(S.srec + 8 * (int64_t)s0);
+        if (shifts & 64)
+            run_item<MODE, true>(P, S, p0, np, shifts, shp, ns, hist, T, X, lane);
         else
-            run_item<MODE, false>(P, S, it, sh, ns, s0, hist, T, X, lane);
+            run_item<MODE, false>(P, S, p0, np, shifts, shp, ns, hist, T, X, lane);
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     if (Sp) flush_hist(P, hist, *Sp, cur_rs, cur_rp, lane);
@@ -703,6 +744,7 @@ void count_fast(Ctx &ctx, std::vector<FastSample> &samples, const mia_bins &bins
         S.pa = F.pos.a; S.pb = F.pos.b; S.pl = F.pos.l; S.pw = F.pos.w;
         S.sa = F.shp.a; S.sb = F.shp.b; S.sl = F.shp.l;
         S.sax = F.shp.ax; S.say = F.shp.ay; S.se = F.shp.e; S.sw = F.shp.w;
+        S.srec = F.shp.rec;
         S.pstart = F.pos.start;
         S.L = F.g.L;
         S.half = F.g.half;
@@ -742,31 +784,62 @@ void count_fast(Ctx &ctx, std::vector<FastSample> &samples, const mia_bins &bins
     A.nbins = B;
     A.T = T;
     A.X = X;
-    A.lg0 = (float)std::log2(bins.r2_edges[0]);
-    A.lgs = (float)(n_r / std::log2(bins.r2_edges[n_r] / bins.r2_edges[0]));
+    // FP32 bin guesses and their guard bands (bin units).  The guess error
+    // is bounded by a few FP32 ulps of the log2 / linear transform (see the
+    // bounds below, with a 20x margin); rechecks happen only inside the band.
+    A.T0 = bins.r2_edges[0];
+    A.Tn = bins.r2_edges[n_r];
+    double l0 = std::log2(bins.r2_edges[0]), ln = std::log2(bins.r2_edges[n_r]);
+    double lgs = n_r / (ln - l0);
+    A.lg0 = (float)l0;
+    A.lgs = (float)lgs;
+    const double u = std::ldexp(1.0, -23);  // FP32 ulp of 1
+    double err_r = (4 * u * std::fmax(std::fabs(l0), std::fabs(ln)) + 4 * u) * lgs + 2 * u * n_r;
+    double err_2;
     if (bins.mode == MIA_RPPI) {
+        A.X0 = bins.x2_edges[0];
+        A.Xn = bins.x2_edges[n_2];
         A.x0 = (float)bins.x2_edges[0];
-        A.xs = (float)(n_2 / (bins.x2_edges[n_2] - bins.x2_edges[0]));
+        double xs = n_2 / (bins.x2_edges[n_2] - bins.x2_edges[0]);
+        A.xs = (float)xs;
+        double mx = std::fmax(std::fabs(bins.x2_edges[0]), std::fabs(bins.x2_edges[n_2]));
+        err_2 = 4 * u * mx * xs + 2 * u * n_2;
     } else {
+        A.X0 = 0;
+        A.Xn = 0;
         A.x0 = -1.0f;
         A.xs = (float)(n_2 / 2.0);
+        err_2 = 6 * u * n_2;
     }
+    A.dr = (float)std::fmax(1e-4, 20 * err_r);
+    A.d2 = (float)std::fmax(1e-4, 20 * err_2);
+    A.exact_only = (A.dr > 0.05f || A.d2 > 0.05f) ? 1 : 0;
     A.samples = sdd;
-    int off = 4 * B;
+    // histogram copies: enough LDS for ~5 waves per SIMD, at most 4 copies
+    int copies_log2 = 0;
+    {
+        size_t per_copy = (size_t)B * (4 + 16 + (any_w ? 8 : 0) + (any_var ? 8 : 0));
+        int want = tuning && tuning->hist_copies > 0 ? tuning->hist_copies : 2;
+        while ((1 << (copies_log2 + 1)) <= want && (1 << (copies_log2 + 1)) <= 4 &&
+               per_copy * (2u << copies_log2) * 20 <= 160 * 1024)
+            ++copies_log2;
+    }
+    A.copies_log2 = copies_log2;
+    const int BC = B << copies_log2;
+    int off = 4 * BC;
     off = (off + 7) & ~7;
     A.off_sp = off;
-    off += 8 * B;
+    off += 8 * BC;
     A.off_sx = off;
-    off += 8 * B;
+    off += 8 * BC;
     A.off_dd = off;
-    if (A.weighted) off += 8 * B;
+    if (A.weighted) off += 8 * BC;
     A.off_var = off;
-    if (A.var) off += 8 * B;
+    if (A.var) off += 8 * BC;
     A.hist_stride = (off + 15) & ~15;
     A.rechecks = rechecks;
     size_t thr_bytes = ((size_t)(n_r + 1 + n_2 + 1) * 8 + 15) & ~(size_t)15;
-    size_t lds = thr_bytes + sizeof(ShapeLds) * kMaxBlockShapes * kWavesPerBlock +
-                 (size_t)A.hist_stride * kWavesPerBlock;
+    size_t lds = thr_bytes + (size_t)A.hist_stride * kWavesPerBlock;
     MIA_REQUIRE(lds <= 160 * 1024, "histogram too large for LDS (n_r*n_2 too big)");
     int blocks_per_cu = (int)std::max<size_t>(1, std::min<size_t>(8, (160 * 1024) / lds));
     int waves = tuning && tuning->waves > 0 ? tuning->waves

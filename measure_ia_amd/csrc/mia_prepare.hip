@@ -153,9 +153,20 @@ __global__ void k_scatter(const double *__restrict__ xyz, int64_t n, Grid g,
     out.reg[d] = reg[i];
     out.key[d] = k;
     if (axis) {
-        out.ax[d] = axis[2 * i];
-        out.ay[d] = axis[2 * i + 1];
-        out.e[d] = e[i];
+        double ax = axis[2 * i], ay = axis[2 * i + 1], ei = e[i];
+        double wi = w ? w[i] : 1.0;
+        out.ax[d] = ax;
+        out.ay[d] = ay;
+        out.e[d] = ei;
+        double *r = out.rec + 8 * d;
+        r[0] = out.a[d];
+        r[1] = out.b[d];
+        r[2] = out.l[d];
+        r[3] = ax;
+        r[4] = ay;
+        r[5] = ei * wi;
+        r[6] = wi;
+        r[7] = ei;
     }
 }
 
@@ -284,6 +295,7 @@ static void sort_one(Ctx &ctx, const double *xyz, int64_t n, const double *w, co
         out.ax = ctx.arena.get<double>(tag + "/ax", nn);
         out.ay = ctx.arena.get<double>(tag + "/ay", nn);
         out.e = ctx.arena.get<double>(tag + "/e", nn);
+        out.rec = ctx.arena.get<double>(tag + "/rec", 8 * nn);
     }
     out.start = ctx.arena.get<int32_t>(tag + "/start", (size_t)keys + 1);
     int32_t *counts = ctx.arena.get<int32_t>(tag + "/counts", (size_t)keys + 1);
