@@ -1,0 +1,174 @@
+"""Pin the oracle's pair counters (oracle/brute.py, oracle/tree.py) without the reference.
+
+The raw catalogue behind the reference's pair-level tests is not shipped, so
+the counters are pinned by checks a plausible slip would fail:
+  * an independent per-pair Python loop (math.* scalars) on tiny inputs;
+  * hand-placed pairs with known (r_p, pi) and axes at 0, 45, 90 degrees,
+    where e+ = +e, 0, -e (the box convention, measure_IA_base.py:226);
+  * uniform randoms: DD per bin equals the analytic RR within Poisson error;
+  * weights 0.5 -> DD and S+D divided by 4 (tests/test_weights.py:5-84);
+  * jackknife: realisation i equals a full count with region i removed;
+  * the KDTree counter equals the brute counter.
+"""
+import numpy as np
+import pytest
+
+from oracle import brute, post, prep, tree
+
+
+def _sample(n_pos, n_shape, L, seed, los=2, weights=False):
+    rng = np.random.default_rng(seed)
+    d = {"Position": rng.uniform(0, L, (n_pos, 3)), "Position_shape_sample": rng.uniform(0, L, (n_shape, 3)),
+         "Axis_Direction": rng.normal(size=(n_shape, 2)), "LOS": los,
+         "q": rng.uniform(0.3, 1.0, n_shape)}
+    if weights:
+        d["weight"] = rng.uniform(0.5, 1.5, n_pos)
+        d["weight_shape_sample"] = rng.uniform(0.5, 1.5, n_shape)
+    return d
+
+
+@pytest.mark.parametrize("mode", ["rppi", "rmu"])
+@pytest.mark.parametrize("los", [0, 1, 2])
+def test_brute_equals_scalar_loop(mode, los):
+    L = 10.0
+    d = _sample(60, 40, L, seed=los, los=los, weights=True)
+    pr = prep.prepare(d)
+    r_bins, pi_bins, mu_bins = prep.bin_edges(0.2, 4.0, 5, 6, 3.0, L)
+    sb = pi_bins if mode == "rppi" else mu_bins
+    ref = brute.count_pairs(pr, r_bins, sb, 0.2, 4.0, L, mode=mode)
+    sc = brute.count_pairs_scalar(pr, r_bins, sb, 0.2, 4.0, L, mode=mode)
+    R = pr["R"]
+    got = np.zeros_like(ref["DD"])
+    gp = np.zeros_like(ref["DD"])
+    gx = np.zeros_like(ref["DD"])
+    for (i, j), (dd, ep, ex) in sc.items():
+        got[i, j], gp[i, j], gx[i, j] = dd, ep, ex
+    assert ref["npairs"] > 50
+    np.testing.assert_allclose(ref["DD"], got, rtol=1e-13)
+    np.testing.assert_allclose(ref["SplusD"], gp / (2 * R), rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(ref["ScrossD"], gx / (2 * R), rtol=1e-9, atol=1e-12)
+
+
+def _known_case():
+    """One shape at the box centre, positions at chosen separations."""
+    L = 20.0
+    c = np.array([10.0, 10.0, 10.0])
+    # (dx, dy, dz) from position to shape: shape - position = sep
+    seps = np.array([[1.5, 0.0, 0.5],     # r_p 1.5 along x
+                     [0.0, 3.0, -2.5],    # r_p 3 along y
+                     [2.0, 2.0, 1.0],     # r_p 2.83 at 45 deg
+                     [0.05, 0.0, 0.0],    # r_p below r_min: excluded
+                     [9.0, 0.0, 0.0]])    # beyond r_max: excluded
+    pos = c - seps
+    return L, c, pos
+
+
+@pytest.mark.parametrize("angle, expect", [(0.0, [1, -1, 0]), (np.pi / 2, [-1, 1, 0]),
+                                           (np.pi / 4, [0, 0, 1])])
+def test_hand_placed_known_answers(angle, expect):
+    L, c, pos = _known_case()
+    q = 0.5
+    e = (1 - q ** 2) / (1 + q ** 2)
+    d = {"Position": pos, "Position_shape_sample": c[None, :], "LOS": 2, "q": np.array([q]),
+         "Axis_Direction": np.array([[np.cos(angle), np.sin(angle)]])}
+    pr = prep.prepare(d)
+    r_bins, pi_bins, _ = prep.bin_edges(0.1, 5.0, 4, 4, 4.0, L)
+    out = brute.count_pairs(pr, r_bins, pi_bins, 0.1, 5.0, L)
+    assert out["npairs"] == 3
+    # bins: r edges 0.1, 0.266, 0.707, 1.88, 5 ; pi edges -4, -2, 0, 2, 4
+    exp_bins = [(2, 2), (3, 0), (3, 2)]
+    for (ir, ip), sgn in zip(exp_bins, expect):
+        assert out["DD"][ir, ip] == 1.0
+        assert out["SplusD"][ir, ip] * 2 * pr["R"] == pytest.approx(sgn * e, abs=1e-12)
+    assert out["DD"].sum() == 3
+    # a planted sign slip (the lightcone convention -e cos 2phi) fails this check
+    if expect[0] != 0:
+        assert not np.isclose(-out["SplusD"][2, 2] * 2 * pr["R"], expect[0] * e)
+
+
+def test_uniform_dd_matches_analytic_rr():
+    L = 40.0
+    rng = np.random.default_rng(7)
+    n = 3000
+    d = {"Position": rng.uniform(0, L, (n, 3)), "Position_shape_sample": rng.uniform(0, L, (n, 3)),
+         "Axis_Direction": rng.normal(size=(n, 2)), "LOS": 2, "q": rng.uniform(0.3, 1, n)}
+    pr = prep.prepare(d)
+    r_bins, pi_bins, mu_bins = prep.bin_edges(0.5, 8.0, 4, 4, 8.0, L)
+    out = brute.count_pairs(pr, r_bins, pi_bins, 0.5, 8.0, L)
+    rr = post.rr_grid("rppi", r_bins, pi_bins, L ** 3, n, n)
+    z = (out["DD"] - rr) / np.sqrt(rr)
+    assert np.abs(z).max() < 5, z
+    # (r, mu): uniform pairs are uniform in mu; the expectation is the spherical
+    # shell times dmu/2.  (The reference's own RR uses spherical-cap volumes,
+    # measure_IA_base.py:274-351, which is not uniform in mu; the fixtures pin
+    # that form, see test_oracle_post.py.)
+    out = brute.count_pairs(pr, r_bins, mu_bins, 0.5, 8.0, L, mode="rmu")
+    shell = 4 * np.pi / 3 * (r_bins[1:] ** 3 - r_bins[:-1] ** 3)
+    expect = n * n * shell[:, None] * (np.diff(mu_bins) / 2)[None, :] / L ** 3
+    z = (out["DD"] - expect) / np.sqrt(expect)
+    assert np.abs(z).max() < 5, z
+
+
+def test_weight_scaling():
+    L = 15.0
+    d = _sample(300, 200, L, seed=2)
+    r_bins, pi_bins, _ = prep.bin_edges(0.2, 5.0, 5, 4, 5.0, L)
+    a = brute.count_pairs(prep.prepare(d), r_bins, pi_bins, 0.2, 5.0, L)
+    d["weight"] = np.full(300, 0.5)
+    d["weight_shape_sample"] = np.full(200, 0.5)
+    b = brute.count_pairs(prep.prepare(d), r_bins, pi_bins, 0.2, 5.0, L)
+    np.testing.assert_array_equal(a["DD"], 4 * b["DD"])
+    np.testing.assert_allclose(a["SplusD"], 4 * b["SplusD"], rtol=1e-13, atol=1e-15)
+
+
+@pytest.mark.parametrize("mode", ["rppi", "rmu"])
+def test_jackknife_leave_one_out_identity(mode):
+    """DD - DD_jk[i] equals DD of the sample with region i removed (both samples)."""
+    L = 12.0
+    d = _sample(700, 400, L, seed=5)
+    d["Position"][:40] = np.round(d["Position"][:40] / 6.0) * 6.0  # objects on faces -> region 0
+    pr = prep.prepare(d)
+    r_bins, pi_bins, mu_bins = prep.bin_edges(0.2, 5.0, 5, 4, 5.0, L)
+    sb = pi_bins if mode == "rppi" else mu_bins
+    jp, js = prep.jackknife_region_indices(pr["positions"], pr["shapes"], L, 2)
+    full = brute.count_pairs(pr, r_bins, sb, 0.2, 5.0, L, mode=mode, jk_pos=jp, jk_shape=js, num_box=8)
+    for i in range(8):
+        sub = dict(pr)
+        mp, ms = jp != i, js != i
+        for k in ("positions", "weight"):
+            sub[k] = pr[k][mp]
+        for k in ("shapes", "axis", "e", "weight_shape"):
+            sub[k] = pr[k][ms]
+        part = brute.count_pairs(sub, r_bins, sb, 0.2, 5.0, L, mode=mode)
+        np.testing.assert_array_equal(full["DD"] - full["DD_jk"][i], part["DD"])
+        np.testing.assert_allclose(full["SplusD"] * 2 * pr["R"] - full["SplusD_jk"][i],
+                                   part["SplusD"] * 2 * pr["R"], rtol=1e-9, atol=1e-11)
+        # a planted slip -- tallying a pair twice when shape and position share
+        # the region -- adds DD(region i alone) to the tally and breaks the identity
+        if i == 0:
+            only = dict(pr)
+            for k in ("positions", "weight"):
+                only[k] = pr[k][~mp]
+            for k in ("shapes", "axis", "e", "weight_shape"):
+                only[k] = pr[k][~ms]
+            same = brute.count_pairs(only, r_bins, sb, 0.2, 5.0, L, mode=mode)["DD"]
+            assert same.sum() > 0
+            assert not np.array_equal(full["DD"] - (full["DD_jk"][i] + same), part["DD"])
+
+
+@pytest.mark.parametrize("mode", ["rppi", "rmu"])
+def test_tree_equals_brute(mode):
+    L = 30.0
+    d = _sample(1500, 600, L, seed=9, weights=True)
+    pr = prep.prepare(d)
+    r_bins, pi_bins, mu_bins = prep.bin_edges(0.1, 6.0, 6, 6, 6.0, L)
+    sb = pi_bins if mode == "rppi" else mu_bins
+    jp, js = prep.jackknife_region_indices(pr["positions"], pr["shapes"], L, 3)
+    a = brute.count_pairs(pr, r_bins, sb, 0.1, 6.0, L, mode=mode, jk_pos=jp, jk_shape=js, num_box=27)
+    b = tree.count_pairs(pr, r_bins, sb, 0.1, 6.0, L, mode=mode, jk_pos=jp, jk_shape=js, num_box=27,
+                         chunk_size=100, num_nodes=2)
+    np.testing.assert_allclose(a["DD"], b["DD"], rtol=1e-13)
+    np.testing.assert_allclose(a["DD_jk"], b["DD_jk"], rtol=1e-13)
+    np.testing.assert_allclose(a["SplusD"], b["SplusD"], rtol=1e-10, atol=1e-13)
+    np.testing.assert_allclose(a["SplusD_jk"], b["SplusD_jk"], rtol=1e-10, atol=1e-13)
+    assert a["npairs"] == b["npairs"]
