@@ -1,0 +1,227 @@
+"""Benchmark of the MI355X pair-counting engine (one JSON line on stdout).
+
+Workload (default "clustered1M"): 1,000,000 positions in a periodic box of
+L = 205 Mpc/h drawn as 5,000 Gaussian clusters (sigma = 1, Poisson(150)
+members) plus a uniform background; a random 200,000-galaxy shape subset with
+ellipticity aligned to the cluster centre (A = 0.1, noise N(0, 0.2));
+w_gg + w_g+ binning: 10 log r_p bins over 0.1-20, 20 linear pi bins over
+|pi| < 60, 27 jackknife regions.  Synthetic data, fixed seed.
+
+A step is one full jackknife count of the sample (region ids, cell sort,
+work plan, pair kernel) with the inputs already in HBM, plus the copy of the
+histograms to the host.  ``value`` = in-domain pairs / second over the timed
+steps.  The pair kernel's own time comes from HIP events the library records
+on the launch stream.  Beside it: the plain FP64 GPU path (one thread per
+shape, global atomics) timed on a seeded 2% subset of the shapes, the
+end-to-end measure_xi_w call (host prep + count + estimators + HDF5), and the
+CPU baseline: the oracle's restatement of the reference's KDTree +
+multiprocessing counter timed on a seeded sample of shape chunks.
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64_TFLOPS = 78.6      # MI355X dense FP64 vector
+FLOP_PER_PAIR = {"rppi": 28, "rmu": 32}   # algorithmic FLOP per in-domain pair
+
+WORKLOADS = {
+    "clustered1M": dict(cat="clustered1M", mode="rppi", r=(0.1, 20.0), nr=10, n2=20, pi_max=60.0, jk=3,
+                        desc="1,000,000 clustered positions / 200,000 aligned shapes, L=205, 27 JK, "
+                             "10 log r_p bins 0.1-20, 20 pi bins |pi|<60"),
+    "uniform20k": dict(cat="uniform20k", mode="rppi", r=(0.1, 20.0), nr=10, n2=20, pi_max=60.0, jk=3,
+                       desc="20,000 uniform galaxies (shapes = positions), L=205, 27 JK"),
+    "multipoles2M": dict(cat="multipoles2M", mode="rmu", r=(0.1, 60.0), nr=15, n2=20, pi_max=None, jk=4,
+                         desc="2,000,000 clustered positions / 500,000 shapes, L=205, 64 JK, "
+                              "15 log r bins 0.1-60, 20 mu bins"),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_inputs(w):
+    from measure_ia_amd import binning, hostprep, synth
+    from measure_ia_amd.box import MeasureIABox  # noqa: F401
+    cat = synth.config(w["cat"])
+    L = cat.boxsize
+    p = hostprep.prepare(cat.data)
+    r_bins = np.logspace(np.log10(w["r"][0]), np.log10(w["r"][1]), w["nr"] + 1)
+    pi_max = w["pi_max"] if w["pi_max"] is not None else L / 2.0
+    pi_bins = np.linspace(-pi_max, pi_max, w["n2"] + 1)
+    mu_bins = np.linspace(-1, 1, w["n2"] + 1)
+    if w["mode"] == "rppi":
+        th = binning.rppi_thresholds(r_bins, pi_bins, *w["r"])
+        sb = pi_bins
+    else:
+        th = binning.rmu_thresholds(r_bins, mu_bins, *w["r"])
+        sb = mu_bins
+    return cat, p, th, r_bins, sb
+
+
+def measured_traffic(workload):
+    """HBM bytes per pair-kernel launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes."""
+    fn = os.path.join(ROOT, "profiles", f"k_pairs_{workload}_traffic.json")
+    if not os.path.exists(fn):
+        return None
+    with open(fn) as f:
+        return float(json.load(f)["bytes_per_launch"])
+
+
+def cpu_baseline(cat, p, w, r_bins, sb, budget_chunks=48):
+    """Oracle tree + multiprocessing counter on a seeded sample of 1000-shape chunks."""
+    from oracle import prep as oprep
+    from oracle import tree
+    workers = min(16, tree.default_workers())
+    pr = oprep.prepare(cat.data)
+    L = cat.boxsize
+    jp, js = oprep.jackknife_region_indices(pr["positions"], pr["shapes"], L, w["jk"])
+    n_chunks = (len(pr["shapes"]) + 999) // 1000
+    rng = np.random.default_rng(12345)
+    pick = sorted(rng.choice(n_chunks, size=min(budget_chunks, n_chunks), replace=False).tolist())
+    out = tree.count_pairs(pr, r_bins, sb, w["r"][0], w["r"][1], L, mode=w["mode"], jk_pos=jp,
+                           jk_shape=js, num_box=w["jk"] ** 3, chunk_size=1000, num_nodes=workers,
+                           chunk_subset=pick)
+    rate = out["npairs"] / out["seconds"]
+    return dict(value=rate, unit="pairs/s", cores=workers, kind="port",
+                sample=f"{len(pick)} of {n_chunks} seeded 1000-shape chunks ({out['npairs']} pairs, "
+                       f"{out['seconds']:.1f} s), KDTree + numpy + Pool restatement of "
+                       f"measure_w_box_jk.py:543-857",
+                seconds_full_estimate=out["seconds"] * n_chunks / len(pick))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="clustered1M", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain GPU comparator")
+    ap.add_argument("--no-e2e", action="store_true", help="skip the end-to-end measure_xi_w call")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run three 512 MiB device copies (the reference kernel for the FETCH_SIZE / "
+                         "WRITE_SIZE scale factors) and exit")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); the counter "
+              f"would shard by sample or shape-cell slab with one histogram all-reduce, which is not "
+              f"built.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    if a.copy_calibration:
+        n = 64 * 1024 * 1024
+        x = torch.ones(n, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": n * 8, "copies": 3}))
+        return
+    from measure_ia_amd.engine import SampleArrays, get_engine
+
+    w = WORKLOADS[a.workload]
+    t0 = time.perf_counter()
+    cat, p, th, r_bins, sb = build_inputs(w)
+    log(f"inputs built in {time.perf_counter() - t0:.1f} s: {p.n_pos} positions, {p.n_shape} shapes")
+    eng = get_engine(0)
+    dev = torch.device("cuda", 0)
+    L = cat.boxsize
+    jk = w["jk"]
+    # inputs resident in HBM before the timed region
+    to = lambda x: torch.from_numpy(np.ascontiguousarray(x)).to(dev)  # noqa: E731
+    s_dev = SampleArrays(to(p.positions), to(p.shapes), to(p.axis), to(p.e), None, None, L, p.los, jk,
+                         (L / 2.0) * 2.0 / jk)
+    torch.cuda.synchronize()
+    for _ in range(a.warmup):
+        res = eng.count(s_dev, th, method="fast")
+    torch.cuda.synchronize()
+    kstats = []
+    t1 = time.perf_counter()
+    for _ in range(a.steps):
+        res = eng.count(s_dev, th, method="fast")
+        kstats.append(res.stats)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t1
+    ms_step = elapsed / a.steps * 1e3
+    pairs = int(res.npairs[0].sum())
+    value = pairs / (ms_step / 1e3)
+    k_ms = float(np.mean([s["ms_pairs"] for s in kstats]))
+    flops = pairs * FLOP_PER_PAIR[w["mode"]]
+    achieved = flops / (k_ms / 1e3) / 1e12
+    log(f"fast: {ms_step:.2f} ms/step, pair kernel {k_ms:.2f} ms, {pairs} pairs")
+
+    plain = None
+    if not a.no_plain:
+        frac = 0.02
+        ns = max(1, int(p.n_shape * frac))
+        sel = np.sort(np.random.default_rng(7).choice(p.n_shape, ns, replace=False))
+        s_pl = SampleArrays(s_dev.positions, to(p.shapes[sel]), to(p.axis[sel]), to(p.e[sel]), None, None,
+                            L, p.los, jk, (L / 2.0) * 2.0 / jk)
+        eng.count(s_pl, th, method="plain")
+        torch.cuda.synchronize()
+        tp = time.perf_counter()
+        rp = eng.count(s_pl, th, method="plain")
+        torch.cuda.synchronize()
+        pl_s = time.perf_counter() - tp
+        pl_pairs = int(rp.npairs[0].sum())
+        plain_rate = pl_pairs / pl_s
+        rf = eng.count(s_pl, th, method="fast")
+        assert np.array_equal(rf.npairs, rp.npairs), "plain and fast counts differ"
+        plain = dict(pairs_per_s=plain_rate, sample=f"seeded {ns} of {p.n_shape} shapes ({pl_pairs} pairs)",
+                     s_per_full_count_est=pairs / plain_rate, kernel_ms=rp.stats["ms_pairs"])
+        log(f"plain: {plain_rate:.3e} pairs/s on {frac:.0%} of shapes")
+
+    e2e = None
+    if not a.no_e2e and w["mode"] == "rppi":
+        from measure_ia_amd import MeasureIABox
+        with tempfile.TemporaryDirectory() as td:
+            fn = os.path.join(td, "bench.hdf5")
+            box = MeasureIABox(cat.data, fn, None, None, list(w["r"]), w["nr"], w["n2"], w["pi_max"],
+                               boxsize=L, verbose=False)
+            box.measure_xi_w("warm", "both", jk ** 3, temp_file_path=None)
+            te = time.perf_counter()
+            box.measure_xi_w("bench", "both", jk ** 3, temp_file_path=None)
+            e2e = time.perf_counter() - te
+        log(f"end-to-end measure_xi_w: {e2e:.3f} s")
+
+    cpu = None
+    if not a.no_cpu:
+        tc = time.perf_counter()
+        cpu = cpu_baseline(cat, p, w, r_bins, sb)
+        log(f"cpu baseline: {cpu['value']:.3e} pairs/s on {cpu['cores']} cores "
+            f"({time.perf_counter() - tc:.1f} s)")
+
+    line = {
+        "metric": "pairs/s & s per wgg+wg+ (1M gal, 27 JK): batched-FP32 vs plain GPU",
+        "value": value, "unit": "pairs/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{a.workload}: {w['desc']}", "mode": w["mode"], "jackknife": jk ** 3,
+                   "positions": int(p.n_pos), "shapes": int(p.n_shape), "pairs": pairs},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": measured_traffic(a.workload),
+                     "kernel": "k_pairs", "kernel_ms": k_ms,
+                     "model": f"{FLOP_PER_PAIR[w['mode']]} FLOP per in-domain pair; FP64 vector peak "
+                              f"(the kernel is vector-ALU + LDS-atomic bound, no matrix work)"},
+        "cpu_baseline": cpu,
+        "plain_gpu": plain,
+        "speedup_vs_plain": (value / plain["pairs_per_s"]) if plain else None,
+        "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
+        "s_per_wgg_wgplus": ms_step / 1e3,
+        "end_to_end_measure_xi_w_s": e2e,
+        "engine": {k: v for k, v in res.stats.items()},
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()
