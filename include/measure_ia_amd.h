@@ -52,9 +52,11 @@ enum mia_mode { MIA_RPPI = 0, MIA_RMU = 1 };
 /* Counting method.
  *   MIA_PLAIN: one thread per shape, FP64 geometry in the reference's op
  *              order, global-atomic histograms (the straightforward GPU form).
- *   MIA_FAST : cell-pair work items sorted by jackknife region pair,
- *              per-wavefront LDS histograms in fixed point, FP32 shape
- *              geometry with FP64 separations and FP64 edge rechecks. */
+ *   MIA_FAST : cell-pair work items sorted by jackknife region pair, a
+ *              persistent cost-balanced grid, per-wavefront LDS histograms
+ *              (u32 counts, FP64 sums); FP64 separations and e+ in the
+ *              reference's rounding, FP32 bin guesses resolved exactly in
+ *              FP64 near an edge. */
 enum mia_method { MIA_PLAIN = 0, MIA_FAST = 1 };
 
 typedef struct mia_ctx mia_ctx;
@@ -134,7 +136,7 @@ typedef struct mia_stats {
     double ms_prepare;      /* region ids + cell sort                                  */
     double ms_plan;         /* shape blocks + work items (MIA_FAST)                    */
     double ms_pairs;        /* pair kernel                                             */
-    double ms_finalize;     /* fixed point -> double                                   */
+    double ms_finalize;     /* work after the pair kernel (currently none: ~0)          */
     int64_t candidates;     /* (shape, position) pairs evaluated                       */
     int64_t items;          /* work items (MIA_FAST)                                   */
     int64_t rechecks;       /* pairs re-evaluated in FP64 near an edge (MIA_FAST)      */
