@@ -125,14 +125,16 @@ int mia_count(mia_ctx *ctx, const mia_sample *sample, const mia_bins *bins, cons
         if (stats) std::memset(stats, 0, sizeof(*stats));
         mia::Grid g = mia::make_grid(*sample, *bins, tuning);
         mia::Sorted pos, shp;
-        mia::Sums sums;
+        int *oor = ctx->c.arena.get<int>("oor", 1);
         hipEvent_t e0, e1;
         MIA_HIP(hipEventCreate(&e0));
         MIA_HIP(hipEventCreate(&e1));
         MIA_HIP(hipEventRecord(e0, st));
-        mia::prepare_sample(ctx->c, *sample, g, pos, shp, out->region_stats, sums, st, "s0");
+        mia::fill<int32_t>(oor, 0, 1, st);
+        mia::prepare_sample(ctx->c, *sample, g, pos, shp, out->region_stats, oor,
+                            method == MIA_PLAIN, st, "s0");
         MIA_HIP(hipEventRecord(e1, st));
-        MIA_HIP(hipEventSynchronize(e1));
+        mia::check_prepared(oor, 1, st);
         float ms = 0;
         MIA_HIP(hipEventElapsedTime(&ms, e0, e1));
         (void)hipEventDestroy(e0);
@@ -147,7 +149,6 @@ int mia_count(mia_ctx *ctx, const mia_sample *sample, const mia_bins *bins, cons
             v[0].g = g;
             v[0].pos = pos;
             v[0].shp = shp;
-            v[0].sums = sums;
             v[0].out = out;
             v[0].weighted = weighted;
             mia::count_fast(ctx->c, v, *bins, tuning, st, stats);
@@ -172,10 +173,12 @@ int mia_count_batch(mia_ctx *ctx, int32_t n, const mia_sample *samples, const mi
         hipStream_t st = (hipStream_t)stream;
         if (stats) std::memset(stats, 0, sizeof(*stats));
         std::vector<mia::FastSample> v(n);
+        int *oor = ctx->c.arena.get<int>("oor", n);
         hipEvent_t e0, e1;
         MIA_HIP(hipEventCreate(&e0));
         MIA_HIP(hipEventCreate(&e1));
         MIA_HIP(hipEventRecord(e0, st));
+        mia::fill<int32_t>(oor, 0, n, st);
         for (int i = 0; i < n; ++i) {
             check_sample(&samples[i]);
             bool weighted = samples[i].pos_w != nullptr || samples[i].shp_w != nullptr;
@@ -185,10 +188,10 @@ int mia_count_batch(mia_ctx *ctx, int32_t n, const mia_sample *samples, const mi
             v[i].out = &outs[i];
             v[i].weighted = weighted;
             mia::prepare_sample(ctx->c, samples[i], v[i].g, v[i].pos, v[i].shp,
-                                outs[i].region_stats, v[i].sums, st, "s" + std::to_string(i));
+                                outs[i].region_stats, oor + i, false, st, "s" + std::to_string(i));
         }
         MIA_HIP(hipEventRecord(e1, st));
-        MIA_HIP(hipEventSynchronize(e1));
+        mia::check_prepared(oor, n, st);
         float ms = 0;
         MIA_HIP(hipEventElapsedTime(&ms, e0, e1));
         (void)hipEventDestroy(e0);

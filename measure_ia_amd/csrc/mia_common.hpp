@@ -95,13 +95,8 @@ struct Sorted {
     int32_t *reg = nullptr;                           // jackknife region (0 without jk)
     int32_t *key = nullptr;                           // cell key
     double *ax = nullptr, *ay = nullptr, *e = nullptr; // shapes only
-    double *rec = nullptr;  // shapes only: 8 doubles per shape {a, b, l, ax, ay, e*w, w, e}
+    double *rec = nullptr;  // shapes only: 8 doubles per shape {a, b, l, ax, ay, e*w, w, 2*e*w}
     int32_t *start = nullptr;                         // keys+1 offsets
-};
-
-struct Sums {
-    double wmax_pos = 1, wmax_shape = 1;
-    int out_of_range = 0;
 };
 
 struct Ctx;
@@ -109,8 +104,14 @@ struct Ctx;
 // Host-side helpers implemented in the .hip files.
 void region_ids(Ctx &ctx, const double *xyz, int64_t n, double jk_sub, int jk, int32_t *region,
                 hipStream_t st);
+// Region ids + counting sort of one sample (stream-ordered, no host sync).
+// Sets *oor (device) to 1 if a coordinate lies outside [0, L] or is NaN;
+// check_prepared() reads the flags of n samples back and throws MIA_E_RANGE.
+// soa: also write the structure-of-arrays copies the plain counter reads.
 void prepare_sample(Ctx &ctx, const mia_sample &s, Grid &g, Sorted &pos, Sorted &shp,
-                    double *region_stats, Sums &sums, hipStream_t st, const std::string &tag);
+                    double *region_stats, int *oor, bool soa, hipStream_t st,
+                    const std::string &tag);
+void check_prepared(const int *oor_dev, int n, hipStream_t st);
 void count_plain(Ctx &ctx, const mia_sample &s, const mia_bins &bins, const Grid &g,
                  const Sorted &pos, const Sorted &shp, const mia_counts &out, bool weighted,
                  hipStream_t st, mia_stats *stats);
@@ -119,7 +120,6 @@ struct FastSample {
     const mia_sample *s;
     Grid g;
     Sorted pos, shp;
-    Sums sums;
     const mia_counts *out;
     bool weighted;
 };

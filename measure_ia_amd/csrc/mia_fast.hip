@@ -10,8 +10,9 @@
 //               position region R_p and one periodic shift per axis.  Items
 //               are counting-sorted by (sample, R_s, R_p).
 //   2. pairs  : a persistent grid of wavefronts takes contiguous runs of the
-//               sorted list, balanced by pair count.  Each wave caches its
-//               shape block in LDS and streams positions through registers,
+//               sorted list, balanced by pair count.  Each wave reads its
+//               shape block through the scalar cache (one 64-byte record per
+//               shape, wave-uniform) and streams positions through registers,
 //               64 at a time.  Separations are the reference's own FP64
 //               differences, so r_p^2 (or r^2) and pi are compared exactly
 //               with the host-bisected thresholds (an FP32 log2 only guesses
@@ -36,8 +37,7 @@ namespace {
 
 constexpr int kWavesPerBlock = 4;
 constexpr int kThreads = kWave * kWavesPerBlock;
-constexpr float kMuBand = 2e-6f;     // |mu32 - mu64| bound with margin
-constexpr float kSin2Exact = 1e-10f; // FP64 arccos-domain check below this sin^2
+constexpr double kSin2Exact = 1e-10;  // FP64 arccos-domain check below this sin^2
 
 struct Block {
     int32_t s0, ns;
@@ -59,7 +59,6 @@ static_assert(sizeof(Block) == 24, "Block layout");
 struct SampleDev {
     // sorted sample
     const double *pa, *pb, *pl, *pw;
-    const double *sa, *sb, *sl, *sax, *say, *se, *sw;
     const double *srec;          // 8 doubles per shape, sorted order
     const int32_t *pstart;
     double L, half;
@@ -85,7 +84,9 @@ struct PlanArgs {
     int32_t *key_cursor; // emit pass
     const int32_t *key_off;
     Item *items;
+    int64_t *cost;       // emit pass: ns * np of every item
     int emit;
+    int local;           // 1: aggregate per workgroup in LDS (K1^2 keys fit)
 };
 
 __device__ __forceinline__ int region_of(const Grid &g, int ia, int ib, int il) {
@@ -133,11 +134,13 @@ __device__ __forceinline__ void axis_class(double lo, double hi, double L, doubl
         cls = 3;
 }
 
-// Enumerate (count or emit) the work items of one (block, stencil column).
-__global__ void k_plan_items(PlanArgs P) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int64_t total = (int64_t)P.nblocks * P.nxw * P.nxw;
-    if (t >= total) return;
+// Enumerate the position ranges of one (block, stencil column) pair: slab
+// runs within reach of the block, merged while the periodic shift class and
+// the jackknife slab stay the same, plus the column's anomalous tail.  For
+// each range calls emit(local key, p_begin, p_end, rp, shift byte, block, ns)
+// with local key = R_s * K1 + R_p.
+template <class F>
+__device__ __forceinline__ void plan_ranges(const PlanArgs &P, int64_t t, F &&emit) {
     const Grid &g = P.g;
     int bl = (int)(t / (P.nxw * P.nxw));
     int so = (int)(t % (P.nxw * P.nxw));
@@ -176,33 +179,17 @@ __global__ void k_plan_items(PlanArgs P) {
     }
     int col = jx * g.nc + jy;
     int64_t base = (int64_t)col * (g.nz + 1);
-    int rs = B.rs;
-    int key_base = (B.sample * P.K1 + rs) * P.K1;
-    // walk slabs, merging consecutive included slabs with equal (shift class, jk slab)
-    int run_k0 = -1, run_cls = 0, run_jz = 0;
-    auto emit_range = [&](int p_begin, int p_end, int rp, int cz) {
+    const int blk = P.blk_base + bl;
+    const int kr = B.rs * P.K1;
+    auto range = [&](int p_begin, int p_end, int rp, int cz) {
         if (p_end <= p_begin) return;
         bool wrap = (ca == 3) || (cb == 3) || (cz == 3);
         uint8_t sh = (uint8_t)((ca == 3 ? 0 : ca) | ((cb == 3 ? 0 : cb) << 2) |
                                ((cz == 3 ? 0 : cz) << 4) | (wrap ? 64 : 0));
-        int key = key_base + rp;
-        for (int p = p_begin; p < p_end; p += P.pmax) {
-            int np = p_end - p < P.pmax ? p_end - p : P.pmax;
-            if (!P.emit) {
-                atomicAdd(&P.key_count[key], 1);
-            } else {
-                int slot = P.key_off[key] + atomicAdd(&P.key_cursor[key], 1);
-                Item it;
-                it.blk = P.blk_base + bl;
-                it.p0 = p;
-                it.np = np;
-                it.rp = (int16_t)rp;
-                it.shifts = sh;
-                it.pad = 0;
-                P.items[slot] = it;
-            }
-        }
+        emit(kr + rp, p_begin, p_end, rp, sh, blk, B.ns);
     };
+    // walk slabs, merging consecutive included slabs with equal (shift class, jk slab)
+    int run_k0 = -1, run_cls = 0, run_jz = 0;
     double rz = P.reach_z + 4 * m;
     for (int k = 0; k <= g.nz; ++k) {
         bool inc = false;
@@ -216,9 +203,7 @@ __global__ void k_plan_items(PlanArgs P) {
         }
         int jz = k / g.nzj;
         if (run_k0 >= 0 && (!inc || cz != run_cls || jz != run_jz)) {
-            // close run [run_k0, k)
-            int rp = region_of(g, jx, jy, run_k0);
-            emit_range(P.pstart[base + run_k0], P.pstart[base + k], rp, run_cls);
+            range(P.pstart[base + run_k0], P.pstart[base + k], region_of(g, jx, jy, run_k0), run_cls);
             run_k0 = -1;
         }
         if (inc && run_k0 < 0) {
@@ -228,7 +213,69 @@ __global__ void k_plan_items(PlanArgs P) {
         }
     }
     // anomalous tail of the target column: region 0, any line-of-sight value
-    emit_range(P.pstart[base + g.nz], P.pstart[base + g.nz + 1], 0, 3);
+    range(P.pstart[base + g.nz], P.pstart[base + g.nz + 1], 0, 3);
+}
+
+constexpr int kPlanThreads = 256;
+
+// Count (emit == 0) or emit the work items of every (block, stencil column)
+// pair, counting-sorted by key (sample, R_s, R_p).  With `local` the key
+// counts are aggregated per workgroup in LDS, so each key sees one global
+// atomic per workgroup instead of one per range; the emit pass enumerates the
+// ranges twice (count, then place) around that reservation.
+__global__ __launch_bounds__(kPlanThreads) void k_plan_items(PlanArgs P) {
+    extern __shared__ int32_t lkey[];  // local: counts [nk], reserved bases [nk]
+    const int nk = P.K1 * P.K1;
+    const int kb = P.sample * nk;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = t < (int64_t)P.nblocks * P.nxw * P.nxw;
+    const bool local = P.local != 0;
+    const int pmax = P.pmax;
+    if (local) {
+        for (int j = threadIdx.x; j < 2 * nk; j += blockDim.x) lkey[j] = 0;
+        __syncthreads();
+    }
+    if (!P.emit || local) {
+        if (active)
+            plan_ranges(P, t, [&](int k, int pb, int pe, int, uint8_t, int, int) {
+                int n = (pe - pb + pmax - 1) / pmax;
+                if (local)
+                    atomicAdd(&lkey[k], n);
+                else
+                    atomicAdd(&P.key_count[kb + k], n);
+            });
+    }
+    if (local) {
+        __syncthreads();
+        for (int j = threadIdx.x; j < nk; j += blockDim.x) {
+            int c = lkey[j];
+            if (!P.emit) {
+                if (c) atomicAdd(&P.key_count[kb + j], c);
+            } else {
+                lkey[nk + j] = c ? atomicAdd(&P.key_cursor[kb + j], c) : 0;
+                lkey[j] = 0;
+            }
+        }
+        __syncthreads();
+    }
+    if (!P.emit || !active) return;
+    plan_ranges(P, t, [&](int k, int pb, int pe, int rp, uint8_t sh, int blk, int ns) {
+        int n = (pe - pb + pmax - 1) / pmax;
+        int slot = P.key_off[kb + k] +
+                   (local ? lkey[nk + k] + atomicAdd(&lkey[k], n) : atomicAdd(&P.key_cursor[kb + k], n));
+        for (int q = 0; q < n; ++q) {
+            int p = pb + q * pmax;
+            Item it;
+            it.blk = blk;
+            it.p0 = p;
+            it.np = pe - p < pmax ? pe - p : pmax;
+            it.rp = (int16_t)rp;
+            it.shifts = sh;
+            it.pad = 0;
+            P.items[slot + q] = it;
+            P.cost[slot + q] = (int64_t)ns * it.np;
+        }
+    });
 }
 
 __global__ void k_blocks_count(const int32_t *sstart, Grid g, int ng, int G, int bsz,
@@ -273,13 +320,6 @@ __global__ void k_blocks_emit(const int32_t *sstart, Grid g, int ng, int G, int 
     }
 }
 
-__global__ void k_item_cost(const Item *items, const Block *blocks, int64_t n, int64_t *cost) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    Item it = items[i];
-    cost[i] = (int64_t)blocks[it.blk].ns * it.np;
-}
-
 // ---------------------------------------------------------------------------
 // pair kernel
 
@@ -293,26 +333,31 @@ struct PairArgs {
     int mode, n_r, n_2, weighted, var, nbins;
     const double *T, *X;         // exact thresholds (global copy)
     double T0, Tn, X0, Xn;       // domain bounds (rppi: s and pi; rmu: r^2)
-    float lg0, lgs;              // bin guess: (log2(v) - lg0) * lgs
-    float x0, xs;                // pi: (pi - x0) * xs; mu: (mu + 1) * xs
-    float dr, d2;                // FP32 guard bands in bin units
-    int exact_only;              // 1: skip the FP32 screen (pathological binning)
+    float lgs, lgc;              // r bin guess: log2(v) * lgs + lgc
+    float xs, xc;                // pi: pi * xs + xc; mu: (mu + 1) * xs
+    float hr, h2;                // exact path when |frac(guess) - 1/2| > h (h = 1/2 - guard band)
     const SampleDev *samples;
     int hist_stride;             // bytes per wave histogram
     int copies_log2;             // each bin is kept in 2^copies_log2 interleaved copies
-    int off_sp, off_sx, off_dd, off_var;  // byte offsets inside a wave histogram
+    int off_sp, off_dd, off_var; // byte offsets inside a wave histogram
     unsigned long long *rechecks;
 };
 
-// Shape record {a, b, l, ax, ay, e*w, w, e} read through the scalar cache: the
-// inner loop is uniform over shapes, so every field lands in SGPRs.
+// Wave histogram layout (C = 2^copies_log2 interleaved copies of B bins;
+// slot h = bin * C + copy):
+//   u32     cnt[B*C]                at 0
+//   double2 {S+D, SxD}[B*C]         at off_sp   (one address, offsets 0 / 8)
+//   double  dd[B*C]                 at off_dd   (weighted samples)
+//   double  var[B*C]                at off_var  (variance requested)
+
+// Shape record {a, b, l, ax, ay, e*w, w, 2*e*w} read through the scalar cache:
+// the inner loop is uniform over shapes, so every field lands in SGPRs.
 typedef const __attribute__((address_space(4))) double *ShapePtr;
 
 __device__ void flush_hist(const PairArgs &P, char *hist, const SampleDev &S, int rs, int rp,
                            int lane) {
     uint32_t *cnt = (uint32_t *)hist;
-    double *sp = (double *)(hist + P.off_sp);
-    double *sx = (double *)(hist + P.off_sx);
+    double *spx = (double *)(hist + P.off_sp);
     double *dd = P.weighted ? (double *)(hist + P.off_dd) : nullptr;
     double *vr = P.var ? (double *)(hist + P.off_var) : nullptr;
     const int B = P.nbins;
@@ -323,11 +368,11 @@ __device__ void flush_hist(const PairArgs &P, char *hist, const SampleDev &S, in
         for (int k = 0; k < C; ++k) {
             int h = b * C + k;
             c += cnt[h];
-            vsp += sp[h];
-            vsx += sx[h];
+            vsp += spx[2 * h];
+            vsx += spx[2 * h + 1];
             cnt[h] = 0;
-            sp[h] = 0.0;
-            sx[h] = 0.0;
+            spx[2 * h] = 0.0;
+            spx[2 * h + 1] = 0.0;
             if (dd) {
                 vdd += dd[h];
                 dd[h] = 0.0;
@@ -400,21 +445,26 @@ __device__ __forceinline__ SampleView load_view(const SampleDev *sd) {
     return v;
 }
 
-template <int MODE, bool WRAP>
+// Periodic handling of an item: no shift on any axis, a constant shift per
+// axis, or the reference's per-pair minimum-image wrap.
+enum { SH_NONE = 0, SH_CONST = 1, SH_WRAP = 2 };
+
+template <int MODE, int SH, bool WEIGHTED>
 __device__ __forceinline__ void run_item(const PairArgs &P, const SampleView &S, int p0, int np,
                                          int shifts, ShapePtr shp, int ns, char *hist,
-                                         const double *T, const double *X, int lane) {
+                                         const double *T, const double *X, int lane,
+                                         uint32_t &nre) {
     uint32_t *cnt = (uint32_t *)hist;
-    double *hsp = (double *)(hist + P.off_sp);
-    double *hsx = (double *)(hist + P.off_sx);
+    double *hspx = (double *)(hist + P.off_sp);
     double *hdd = (double *)(hist + P.off_dd);
     double *hvr = (double *)(hist + P.off_var);
     const double L = S.L, half = S.half;
-    const double sha = WRAP ? 0.0 : shift_val(shifts & 3, L);
-    const double shb = WRAP ? 0.0 : shift_val((shifts >> 2) & 3, L);
-    const double shl = WRAP ? 0.0 : shift_val((shifts >> 4) & 3, L);
+    const double sha = SH == SH_CONST ? shift_val(shifts & 3, L) : 0.0;
+    const double shb = SH == SH_CONST ? shift_val((shifts >> 2) & 3, L) : 0.0;
+    const double shl = SH == SH_CONST ? shift_val((shifts >> 4) & 3, L) : 0.0;
     const int n_r = P.n_r, n_2 = P.n_2;
-    const int copy = lane & ((1 << P.copies_log2) - 1);
+    const int cl2 = P.copies_log2;
+    const int copy = lane & ((1 << cl2) - 1);
     // Strided lane -> position map: lane l of batch b takes position b + l*nb,
     // so one batch spans the whole line-of-sight range of the item and its
     // pairs spread over more (r, pi) bins (fewer same-address LDS atomics).
@@ -424,21 +474,21 @@ __device__ __forceinline__ void run_item(const PairArgs &P, const SampleView &S,
         bool valid = pi < np;
         int p = p0 + (valid ? pi : 0);
         double pa = S.pa[p], pb = S.pb[p], pl = S.pl[p];
-        double pw = S.pw[p];
+        double pw = WEIGHTED ? S.pw[p] : 1.0;
         double qa = shp[0], qb = shp[1], ql = shp[2], qax = shp[3], qay = shp[4], qew = shp[5],
-               qw = shp[6];
+               qw = shp[6], q2ew = shp[7];
         for (int j = 0; j < ns; ++j) {
-            // prefetch the next shape record (scalar loads) while this one is used
+            // prefetch the next shape record (one scalar load) while this one is used
             ShapePtr rn = shp + 8 * (j + 1 < ns ? j + 1 : j);
             const double na = rn[0], nbv = rn[1], nl = rn[2], nax = rn[3], nay = rn[4],
-                         new_ = rn[5], nw = rn[6];
+                         new_ = rn[5], nw = rn[6], n2ew = rn[7];
             // the reference's own FP64 separation (measure_w_box_jk.py:159-162)
             double da = qa - pa, db = qb - pb, dl = ql - pl;
-            if (WRAP) {
+            if (SH == SH_WRAP) {
                 da = wrap_sep(da, L, half);
                 db = wrap_sep(db, L, half);
                 dl = wrap_sep(dl, L, half);
-            } else {
+            } else if (SH == SH_CONST) {
                 da = da + sha;
                 db = db + shb;
                 dl = dl + shl;
@@ -454,26 +504,22 @@ __device__ __forceinline__ void run_item(const PairArgs &P, const SampleView &S,
                 rv = r2_column_order(a2, b2, dl * dl, S.los);
                 in = valid & (s > 0.0) & (rv >= P.T0) & (rv < P.Tn);
             }
-            if (__any(in) && in) {
-                // bins: FP32 guess; exact FP64 threshold compare only within the
-                // guard band of an interior edge (the domain edges are exact above)
+            if (in) {
+                // bins: FP32 guess; exact FP64 threshold count when the guess
+                // lies within the guard band of a bin edge (every edge, the
+                // domain edges included, so no clamping is needed)
                 float rv32 = (float)rv;
                 float dlf = (float)dl;
-                float gr = (__log2f(rv32) - P.lg0) * P.lgs;
+                float gr = __fmaf_rn(__log2f(rv32), P.lgs, P.lgc);
                 float g2;
                 if (MODE == MIA_RPPI)
-                    g2 = (dlf - P.x0) * P.xs;
+                    g2 = __fmaf_rn(dlf, P.xs, P.xc);
                 else
                     g2 = __fmaf_rn(dlf, __builtin_amdgcn_rsqf(rv32), 1.0f) * P.xs;
-                int ir = clampi((int)floorf(gr), 0, n_r - 1);
-                int i2 = clampi((int)floorf(g2), 0, n_2 - 1);
-                float fr = gr - (float)ir, f2 = g2 - (float)i2;
-                bool near = ((ir > 0) & (fr < P.dr)) | ((ir < n_r - 1) & (fr > 1.0f - P.dr));
-                if (MODE == MIA_RPPI)
-                    near = near | ((i2 > 0) & (f2 < P.d2)) | ((i2 < n_2 - 1) & (f2 > 1.0f - P.d2));
-                else  // bin 0's lower edge M_0 is a real edge for mu
-                    near = near | (f2 < P.d2) | ((i2 < n_2 - 1) & (f2 > 1.0f - P.d2));
-                if (near | (P.exact_only != 0)) {
+                float fr = floorf(gr), f2 = floorf(g2);
+                int ir = (int)fr, i2 = (int)f2;
+                bool near = (fabsf(gr - fr - 0.5f) > P.hr) | (fabsf(g2 - f2 - 0.5f) > P.h2);
+                if (near) {
                     ir = count_edges(T, n_r, rv);
                     if (MODE == MIA_RPPI) {
                         i2 = count_edges(X, n_2, dl);
@@ -481,7 +527,7 @@ __device__ __forceinline__ void run_item(const PairArgs &P, const SampleView &S,
                         double mu64 = dl / sqrt(rv);
                         i2 = mu64 < X[0] ? n_2 - 1 : count_edges(X, n_2, mu64);
                     }
-                    atomicAdd(P.rechecks, 1ull);
+                    ++nre;
                 }
                 // e+ = e cos 2phi = e (2 cos^2 phi - 1), ex = e sin 2phi = 2 e cos phi |sin phi|
                 // (|.| because phi = arccos(d) lies in [0, pi]); cos phi = t/r_p,
@@ -490,9 +536,10 @@ __device__ __forceinline__ void run_item(const PairArgs &P, const SampleView &S,
                 double c = fma(da, qay, -(db * qax));
                 double r0 = __builtin_amdgcn_rcp(s);
                 double inv = fma(r0, fma(-s, r0, 1.0), r0);
-                double ep = qew * fma(2.0 * inv, t * t, -1.0);
-                double ex = 2.0 * qew * t * fabs(c) * inv;
-                if (c * c * inv < kSin2Exact) {
+                double u = q2ew * t;
+                double ep = fma(u * t, inv, -qew);
+                double ex = (u * fabs(c)) * inv;
+                if (c * c < s * kSin2Exact) {
                     // nearly parallel: the reference's d may round outside [-1, 1],
                     // where arccos gives NaN and the pair adds no e+ (measure_w_box_jk.py:170-175)
                     double rp = sqrt(s);
@@ -503,13 +550,16 @@ __device__ __forceinline__ void run_item(const PairArgs &P, const SampleView &S,
                         ex = 0.0;
                     }
                 }
-                int b = ((ir * n_2 + i2) << P.copies_log2) + copy;
-                double vsp = pw * ep;
+                if (WEIGHTED) {
+                    ep = pw * ep;
+                    ex = pw * ex;
+                }
+                int b = ((__mul24(ir, n_2) + i2) << cl2) | copy;
                 atomicAdd(&cnt[b], 1u);
-                atomicAdd(&hsp[b], vsp);
-                atomicAdd(&hsx[b], pw * ex);
-                if (P.weighted) atomicAdd(&hdd[b], pw * qw);
-                if (P.var) atomicAdd(&hvr[b], vsp * vsp);
+                atomicAdd(&hspx[2 * b], ep);
+                atomicAdd(&hspx[2 * b + 1], ex);
+                if (WEIGHTED) atomicAdd(&hdd[b], pw * qw);
+                if (P.var) atomicAdd(&hvr[b], ep * ep);
             }
             qa = na;
             qb = nbv;
@@ -518,11 +568,12 @@ __device__ __forceinline__ void run_item(const PairArgs &P, const SampleView &S,
             qay = nay;
             qew = new_;
             qw = nw;
+            q2ew = n2ew;
         }
     }
 }
 
-template <int MODE>
+template <int MODE, bool WEIGHTED>
 __global__ __launch_bounds__(kThreads) void k_pairs(PairArgs P) {
     extern __shared__ __align__(16) char lds[];
     const int lane = threadIdx.x & (kWave - 1);
@@ -562,6 +613,7 @@ __global__ __launch_bounds__(kThreads) void k_pairs(PairArgs P) {
     int cur_key_s = -1, cur_rs = -1, cur_rp = -1;
     const SampleDev *Sp = nullptr;
     SampleView S{};
+    uint32_t nre = 0;
     for (int64_t i = it0; i < it1; ++i) {
         const MIA_CONST int32_t *ip = items + 4 * i;
         const int blk = ip[0], p0 = ip[1], np = ip[2], w3 = ip[3];
@@ -583,12 +635,23 @@ __global__ __launch_bounds__(kThreads) void k_pairs(PairArgs P) {
         }
         ShapePtr shp = (ShapePtr)(S.srec + 8 * (int64_t)s0);
         if (shifts & 64)
-            run_item<MODE, true>(P, S, p0, np, shifts, shp, ns, hist, T, X, lane);
+            run_item<MODE, SH_WRAP, WEIGHTED>(P, S, p0, np, shifts, shp, ns, hist, T, X, lane, nre);
+        else if (shifts & 63)
+            run_item<MODE, SH_CONST, WEIGHTED>(P, S, p0, np, shifts, shp, ns, hist, T, X, lane, nre);
         else
-            run_item<MODE, false>(P, S, p0, np, shifts, shp, ns, hist, T, X, lane);
+            run_item<MODE, SH_NONE, WEIGHTED>(P, S, p0, np, shifts, shp, ns, hist, T, X, lane, nre);
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     if (Sp) flush_hist(P, hist, *Sp, cur_rs, cur_rp, lane);
+    if (nre) atomicAdd(P.rechecks, (unsigned long long)nre);
+}
+
+template <int MODE, bool WEIGHTED>
+void launch_pairs(const PairArgs &A, unsigned grid, size_t lds, hipStream_t st) {
+    MIA_HIP(hipFuncSetAttribute((const void *)k_pairs<MODE, WEIGHTED>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((k_pairs<MODE, WEIGHTED>), dim3(grid), dim3(kThreads), lds, st, A);
+    MIA_HIP(hipGetLastError());
 }
 
 inline unsigned nblk(int64_t n, int t) { return (unsigned)((n + t - 1) / t < 1 ? 1 : (n + t - 1) / t); }
@@ -616,16 +679,17 @@ void count_fast(Ctx &ctx, std::vector<FastSample> &samples, const mia_bins &bins
     for (auto &e : ev) MIA_HIP(hipEventCreate(&e));
     MIA_HIP(hipEventRecord(ev[0], st));
 
-    // ---- plan: shape blocks per sample
+    // ---- plan: shape blocks per sample (one host sync for all samples)
     const int bsz = tuning && tuning->block_shapes > 0 ? std::min(tuning->block_shapes, kMaxBlockShapes) : kMaxBlockShapes;
     const int pmax = tuning && tuning->item_positions > 0 ? tuning->item_positions : 1024;
     std::vector<int> blk_base(ns_ + 1, 0), gsz(ns_), ngr(ns_);
+    int32_t *btot = ctx.arena.get<int32_t>("fast/btot", ns_);
     int K1max = 1;
     for (int si = 0; si < ns_; ++si) {
         FastSample &F = samples[si];
         const Grid &g = F.g;
         K1max = std::max(K1max, g.jk > 0 ? g.jk * g.jk * g.jk : 1);
-        // slabs per block: aim for ~bsz shapes per block, dividing the per-jk slab count
+        // slabs per block: aim for ~16 shapes per block, dividing the per-jk slab count
         int G = 1;
         if (tuning && tuning->slabs_per_block > 0) {
             G = tuning->slabs_per_block;
@@ -646,10 +710,13 @@ void count_fast(Ctx &ctx, std::vector<FastSample> &samples, const mia_bins &bins
                            ngr[si], G, bsz, nb);
         MIA_HIP(hipGetLastError());
         exclusive_scan<int32_t>(ctx, nb, nbo, nt, st, "fast/bs" + std::to_string(si), 0);
-        int32_t tot = 0;
-        MIA_HIP(hipMemcpyAsync(&tot, nbo + nt, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        MIA_HIP(hipMemcpyAsync(btot + si, nbo + nt, sizeof(int32_t), hipMemcpyDeviceToDevice, st));
+    }
+    {
+        std::vector<int32_t> h(ns_);
+        MIA_HIP(hipMemcpyAsync(h.data(), btot, sizeof(int32_t) * ns_, hipMemcpyDeviceToHost, st));
         MIA_HIP(hipStreamSynchronize(st));
-        blk_base[si + 1] = blk_base[si] + tot;
+        for (int si = 0; si < ns_; ++si) blk_base[si + 1] = blk_base[si] + h[si];
     }
     const int nblocks = blk_base[ns_];
     Block *blocks = ctx.arena.get<Block>("fast/blocks", (size_t)std::max(nblocks, 1));
@@ -666,6 +733,8 @@ void count_fast(Ctx &ctx, std::vector<FastSample> &samples, const mia_bins &bins
     // ---- plan: work items, counting-sorted by (sample, R_s, R_p)
     const int K1 = K1max;
     const int64_t nkeys = (int64_t)ns_ * K1 * K1;
+    const size_t plan_lds = (size_t)2 * K1 * K1 * sizeof(int32_t);
+    const int plan_local = plan_lds <= 48 * 1024 ? 1 : 0;
     int32_t *kc = ctx.arena.get<int32_t>("fast/keycount", nkeys + 1);
     int32_t *ko = ctx.arena.get<int32_t>("fast/keyoff", nkeys + 1);
     int32_t *kcur = ctx.arena.get<int32_t>("fast/keycur", nkeys + 1);
@@ -696,10 +765,13 @@ void count_fast(Ctx &ctx, std::vector<FastSample> &samples, const mia_bins &bins
         P.key_cursor = kcur;
         P.key_off = ko;
         P.items = nullptr;
+        P.cost = nullptr;
         P.emit = 0;
+        P.local = plan_local;
         int64_t nt = (int64_t)P.nblocks * P.nxw * P.nxw;
         if (nt > 0) {
-            hipLaunchKernelGGL(k_plan_items, dim3(nblk(nt, 256)), dim3(256), 0, st, P);
+            hipLaunchKernelGGL(k_plan_items, dim3(nblk(nt, kPlanThreads)), dim3(kPlanThreads),
+                               plan_local ? plan_lds : 0, st, P);
             MIA_HIP(hipGetLastError());
         }
     }
@@ -709,22 +781,19 @@ void count_fast(Ctx &ctx, std::vector<FastSample> &samples, const mia_bins &bins
     MIA_HIP(hipStreamSynchronize(st));
     const int64_t nitems = nitems32;
     Item *items = ctx.arena.get<Item>("fast/items", (size_t)std::max<int64_t>(nitems, 1));
+    int64_t *cost = ctx.arena.get<int64_t>("fast/cost", nitems + 1);
+    int64_t *cpre = ctx.arena.get<int64_t>("fast/costpre", nitems + 1);
     for (int si = 0; si < ns_; ++si) {
         PlanArgs P = plans[si];
         P.emit = 1;
         P.items = items;
+        P.cost = cost;
         int64_t nt = (int64_t)P.nblocks * P.nxw * P.nxw;
         if (nt > 0) {
-            hipLaunchKernelGGL(k_plan_items, dim3(nblk(nt, 256)), dim3(256), 0, st, P);
+            hipLaunchKernelGGL(k_plan_items, dim3(nblk(nt, kPlanThreads)), dim3(kPlanThreads),
+                               plan_local ? plan_lds : 0, st, P);
             MIA_HIP(hipGetLastError());
         }
-    }
-    int64_t *cost = ctx.arena.get<int64_t>("fast/cost", nitems + 1);
-    int64_t *cpre = ctx.arena.get<int64_t>("fast/costpre", nitems + 1);
-    if (nitems > 0) {
-        hipLaunchKernelGGL(k_item_cost, dim3(nblk(nitems, 256)), dim3(256), 0, st, items, blocks,
-                           nitems, cost);
-        MIA_HIP(hipGetLastError());
     }
     exclusive_scan<int64_t>(ctx, cost, cpre, nitems, st, "fast/cost", 0);
     int64_t total_cost = 0;
@@ -742,8 +811,6 @@ void count_fast(Ctx &ctx, std::vector<FastSample> &samples, const mia_bins &bins
         SampleDev &S = sd[si];
         std::memset(&S, 0, sizeof(S));
         S.pa = F.pos.a; S.pb = F.pos.b; S.pl = F.pos.l; S.pw = F.pos.w;
-        S.sa = F.shp.a; S.sb = F.shp.b; S.sl = F.shp.l;
-        S.sax = F.shp.ax; S.say = F.shp.ay; S.se = F.shp.e; S.sw = F.shp.w;
         S.srec = F.shp.rec;
         S.pstart = F.pos.start;
         S.L = F.g.L;
@@ -785,35 +852,38 @@ void count_fast(Ctx &ctx, std::vector<FastSample> &samples, const mia_bins &bins
     A.T = T;
     A.X = X;
     // FP32 bin guesses and their guard bands (bin units).  The guess error
-    // is bounded by a few FP32 ulps of the log2 / linear transform (see the
-    // bounds below, with a 20x margin); rechecks happen only inside the band.
+    // is bounded by a few FP32 ulps of the log2 / linear transform (the
+    // bounds below, with a 20x margin); the exact FP64 count runs only for
+    // guesses within the band of an edge.
     A.T0 = bins.r2_edges[0];
     A.Tn = bins.r2_edges[n_r];
     double l0 = std::log2(bins.r2_edges[0]), ln = std::log2(bins.r2_edges[n_r]);
     double lgs = n_r / (ln - l0);
-    A.lg0 = (float)l0;
     A.lgs = (float)lgs;
+    A.lgc = (float)(-l0 * lgs);
     const double u = std::ldexp(1.0, -23);  // FP32 ulp of 1
     double err_r = (4 * u * std::fmax(std::fabs(l0), std::fabs(ln)) + 4 * u) * lgs + 2 * u * n_r;
     double err_2;
     if (bins.mode == MIA_RPPI) {
         A.X0 = bins.x2_edges[0];
         A.Xn = bins.x2_edges[n_2];
-        A.x0 = (float)bins.x2_edges[0];
         double xs = n_2 / (bins.x2_edges[n_2] - bins.x2_edges[0]);
         A.xs = (float)xs;
+        A.xc = (float)(-bins.x2_edges[0] * xs);
         double mx = std::fmax(std::fabs(bins.x2_edges[0]), std::fabs(bins.x2_edges[n_2]));
         err_2 = 4 * u * mx * xs + 2 * u * n_2;
     } else {
         A.X0 = 0;
         A.Xn = 0;
-        A.x0 = -1.0f;
         A.xs = (float)(n_2 / 2.0);
+        A.xc = 0.0f;
         err_2 = 6 * u * n_2;
     }
-    A.dr = (float)std::fmax(1e-4, 20 * err_r);
-    A.d2 = (float)std::fmax(1e-4, 20 * err_2);
-    A.exact_only = (A.dr > 0.05f || A.d2 > 0.05f) ? 1 : 0;
+    double dr = std::fmax(1e-4, 20 * err_r), d2 = std::fmax(1e-4, 20 * err_2);
+    // pathological binnings (guard band > 5% of a bin): every pair takes the exact path
+    bool exact_only = dr > 0.05 || d2 > 0.05;
+    A.hr = exact_only ? -1.0f : (float)(0.5 - dr);
+    A.h2 = exact_only ? -1.0f : (float)(0.5 - d2);
     A.samples = sdd;
     // histogram copies: enough LDS for ~5 waves per SIMD, at most 4 copies
     int copies_log2 = 0;
@@ -827,11 +897,9 @@ void count_fast(Ctx &ctx, std::vector<FastSample> &samples, const mia_bins &bins
     A.copies_log2 = copies_log2;
     const int BC = B << copies_log2;
     int off = 4 * BC;
-    off = (off + 7) & ~7;
+    off = (off + 15) & ~15;
     A.off_sp = off;
-    off += 8 * BC;
-    A.off_sx = off;
-    off += 8 * BC;
+    off += 16 * BC;
     A.off_dd = off;
     if (A.weighted) off += 8 * BC;
     A.off_var = off;
@@ -849,15 +917,16 @@ void count_fast(Ctx &ctx, std::vector<FastSample> &samples, const mia_bins &bins
     unsigned grid = (unsigned)((waves + kWavesPerBlock - 1) / kWavesPerBlock);
     if (nitems > 0) {
         if (bins.mode == MIA_RPPI) {
-            MIA_HIP(hipFuncSetAttribute((const void *)k_pairs<MIA_RPPI>,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            hipLaunchKernelGGL(k_pairs<MIA_RPPI>, dim3(grid), dim3(kThreads), lds, st, A);
+            if (A.weighted)
+                launch_pairs<MIA_RPPI, true>(A, grid, lds, st);
+            else
+                launch_pairs<MIA_RPPI, false>(A, grid, lds, st);
         } else {
-            MIA_HIP(hipFuncSetAttribute((const void *)k_pairs<MIA_RMU>,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            hipLaunchKernelGGL(k_pairs<MIA_RMU>, dim3(grid), dim3(kThreads), lds, st, A);
+            if (A.weighted)
+                launch_pairs<MIA_RMU, true>(A, grid, lds, st);
+            else
+                launch_pairs<MIA_RMU, false>(A, grid, lds, st);
         }
-        MIA_HIP(hipGetLastError());
     }
     MIA_HIP(hipEventRecord(ev[2], st));
     MIA_HIP(hipEventRecord(ev[3], st));

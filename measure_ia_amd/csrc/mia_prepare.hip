@@ -90,83 +90,119 @@ __device__ __forceinline__ int axis_cell(double x, int ji, double jk_sub, int pe
     return k;
 }
 
-// Region id, cell key and per-region statistics of every object.
-__global__ void k_classify(const double *__restrict__ xyz, int64_t n, Grid g,
-                           const double *__restrict__ w, const double *__restrict__ e,
-                           int32_t *__restrict__ key, int32_t *__restrict__ reg,
-                           int32_t *__restrict__ counts, double *__restrict__ rstats, int which,
-                           unsigned long long *__restrict__ wmax, int *__restrict__ oor) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double x[3] = {xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};
-    bool ok = true;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) ok = ok && (x[c] >= 0.0) && (x[c] <= g.L);
-    if (!ok) {
-        atomicOr(oor, 1);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) x[c] = 0.0;
+constexpr int kClassifyThreads = 256;
+
+// Region id, cell key and per-region statistics of every object.  With
+// `partials` the region statistics are first summed per workgroup in LDS and
+// written as one row of K*4 partial sums per workgroup (reduced in a fixed
+// order by k_reduce_rows); without it they go straight to global atomics.
+__global__ __launch_bounds__(kClassifyThreads) void k_classify(
+    const double *__restrict__ xyz, int64_t n, Grid g, const double *__restrict__ w,
+    const double *__restrict__ e, int32_t *__restrict__ key, int32_t *__restrict__ reg,
+    int32_t *__restrict__ counts, double *__restrict__ rstats, double *__restrict__ partials,
+    int which, int *__restrict__ oor) {
+    extern __shared__ double rs_l[];
+    const int K4 = g.jk > 0 ? 4 * g.jk * g.jk * g.jk : 0;
+    const bool stats = rstats != nullptr && g.jk > 0;
+    const bool local = stats && partials != nullptr;
+    if (local) {
+        for (int j = threadIdx.x; j < K4; j += blockDim.x) rs_l[j] = 0.0;
+        __syncthreads();
     }
-    int ji[3] = {-1, -1, -1};
-    bool anom = false;
-    int region = 0;
-    if (g.jk > 0) {
+    double *acc = local ? rs_l : rstats;
+    bool bad = false;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        double x[3] = {xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};
+        bool ok = true;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) ji[c] = strict_axis_index(x[c], g.jk_sub, g.jk);
-        anom = ji[0] < 0 || ji[1] < 0 || ji[2] < 0;
-        if (!anom) region = ji[0] * g.jk * g.jk + ji[1] * g.jk + ji[2];
-    }
-    int ca = axis_cell(x[g.ax_a], anom ? -1 : ji[g.ax_a], g.jk_sub, g.ncj, g.cw, g.nc);
-    int cb = axis_cell(x[g.ax_b], anom ? -1 : ji[g.ax_b], g.jk_sub, g.ncj, g.cw, g.nc);
-    int cl = axis_cell(x[g.los], anom ? -1 : ji[g.los], g.jk_sub, g.nzj, g.hz, g.nz);
-    int k = (ca * g.nc + cb) * (g.nz + 1) + (anom ? g.nz : cl);
-    key[i] = k;
-    reg[i] = region;
-    atomicAdd(&counts[k], 1);
-    double wi = w ? w[i] : 1.0;
-    if (g.jk > 0 && rstats) {
-        atomicAdd(&rstats[4 * region + which], 1.0);
-        if (which == 1) {
-            double ei = e[i];
-            atomicAdd(&rstats[4 * region + 2], wi);
-            atomicAdd(&rstats[4 * region + 3], wi * (1 - ei * ei / 2.0));
+        for (int c = 0; c < 3; ++c) ok = ok && (x[c] >= 0.0) && (x[c] <= g.L);
+        if (!ok) {
+            bad = true;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) x[c] = 0.0;
+        }
+        int ji[3] = {-1, -1, -1};
+        bool anom = false;
+        int region = 0;
+        if (g.jk > 0) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) ji[c] = strict_axis_index(x[c], g.jk_sub, g.jk);
+            anom = ji[0] < 0 || ji[1] < 0 || ji[2] < 0;
+            if (!anom) region = ji[0] * g.jk * g.jk + ji[1] * g.jk + ji[2];
+        }
+        int ca = axis_cell(x[g.ax_a], anom ? -1 : ji[g.ax_a], g.jk_sub, g.ncj, g.cw, g.nc);
+        int cb = axis_cell(x[g.ax_b], anom ? -1 : ji[g.ax_b], g.jk_sub, g.ncj, g.cw, g.nc);
+        int cl = axis_cell(x[g.los], anom ? -1 : ji[g.los], g.jk_sub, g.nzj, g.hz, g.nz);
+        int k = (ca * g.nc + cb) * (g.nz + 1) + (anom ? g.nz : cl);
+        key[i] = k;
+        reg[i] = region;
+        atomicAdd(&counts[k], 1);
+        if (stats) {
+            atomicAdd(&acc[4 * region + which], 1.0);
+            if (which == 1) {
+                double wi = w ? w[i] : 1.0;
+                double ei = e[i];
+                atomicAdd(&acc[4 * region + 2], wi);
+                atomicAdd(&acc[4 * region + 3], wi * (1 - ei * ei / 2.0));
+            }
         }
     }
-    double aw = fabs(wi);
-    unsigned long long bits = __double_as_longlong(aw);
-    if (aw == aw) atomicMax(wmax, bits);
+    if (bad) atomicOr(oor, 1);
+    if (local) {
+        __syncthreads();
+        for (int j = threadIdx.x; j < K4; j += blockDim.x)
+            partials[(int64_t)blockIdx.x * K4 + j] = rs_l[j];
+    }
 }
 
+// out[j] += sum over rows r of rows[r*m + j], rows added in order.
+__global__ void k_reduce_rows(const double *__restrict__ rows, int nrows, int m,
+                              double *__restrict__ out) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= m) return;
+    double s = 0.0;
+    for (int r = 0; r < nrows; ++r) s += rows[(int64_t)r * m + j];
+    out[j] += s;
+}
+
+// Counting-sort scatter.  Positions: a, b, l, w (+ reg, key for the plain
+// counter).  Shapes: the 8-double record {a, b, l, ax, ay, e*w, w, 2*e*w}
+// (+ the SoA copies the plain counter reads).
 __global__ void k_scatter(const double *__restrict__ xyz, int64_t n, Grid g,
                           const double *__restrict__ w, const double *__restrict__ axis,
                           const double *__restrict__ e, const int32_t *__restrict__ key,
                           const int32_t *__restrict__ reg, const int32_t *__restrict__ start,
-                          int32_t *__restrict__ cursor, Sorted out) {
+                          int32_t *__restrict__ cursor, Sorted out, int soa) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int k = key[i];
     int64_t d = (int64_t)start[k] + atomicAdd(&cursor[k], 1);
-    out.a[d] = xyz[3 * i + g.ax_a];
-    out.b[d] = xyz[3 * i + g.ax_b];
-    out.l[d] = xyz[3 * i + g.los];
-    out.w[d] = w ? w[i] : 1.0;
-    out.reg[d] = reg[i];
-    out.key[d] = k;
+    double a = xyz[3 * i + g.ax_a], b = xyz[3 * i + g.ax_b], l = xyz[3 * i + g.los];
+    double wi = w ? w[i] : 1.0;
+    if (!axis || soa) {
+        out.a[d] = a;
+        out.b[d] = b;
+        out.l[d] = l;
+        out.w[d] = wi;
+    }
+    if (soa) {
+        out.reg[d] = reg[i];
+        out.key[d] = k;
+    }
     if (axis) {
         double ax = axis[2 * i], ay = axis[2 * i + 1], ei = e[i];
-        double wi = w ? w[i] : 1.0;
-        out.ax[d] = ax;
-        out.ay[d] = ay;
-        out.e[d] = ei;
-        double *r = out.rec + 8 * d;
-        r[0] = out.a[d];
-        r[1] = out.b[d];
-        r[2] = out.l[d];
-        r[3] = ax;
-        r[4] = ay;
-        r[5] = ei * wi;
-        r[6] = wi;
-        r[7] = ei;
+        if (soa) {
+            out.ax[d] = ax;
+            out.ay[d] = ay;
+            out.e[d] = ei;
+        }
+        double ew = ei * wi;
+        double2 *r = (double2 *)(out.rec + 8 * d);
+        r[0] = make_double2(a, b);
+        r[1] = make_double2(l, ax);
+        r[2] = make_double2(ay, ew);
+        r[3] = make_double2(wi, 2.0 * ew);
     }
 }
 
@@ -281,7 +317,7 @@ void region_ids(Ctx &ctx, const double *xyz, int64_t n, double jk_sub, int jk, i
 
 static void sort_one(Ctx &ctx, const double *xyz, int64_t n, const double *w, const double *axis,
                      const double *e, const Grid &g, Sorted &out, double *rstats, int which,
-                     unsigned long long *wmax, int *oor, hipStream_t st, const std::string &tag) {
+                     int *oor, bool soa, hipStream_t st, const std::string &tag) {
     int keys = g.keys();
     out.n = n;
     size_t nn = (size_t)(n > 0 ? n : 1);
@@ -305,43 +341,48 @@ static void sort_one(Ctx &ctx, const double *xyz, int64_t n, const double *w, co
     fill<int32_t>(counts, 0, keys + 1, st);
     fill<int32_t>(cursor, 0, keys + 1, st);
     if (n > 0) {
-        hipLaunchKernelGGL(k_classify, dim3(grid_for(n, 256)), dim3(256), 0, st, xyz, n, g, w, e,
-                           key0, reg0, counts, rstats, which, wmax, oor);
+        const int K4 = g.jk > 0 ? 4 * g.jk * g.jk * g.jk : 0;
+        unsigned grid = std::min<unsigned>(grid_for(n, kClassifyThreads), 4u * ctx.cu_count);
+        // per-workgroup partial region sums when they fit in LDS
+        bool local = rstats && K4 > 0 && K4 * 8 <= 32 * 1024;
+        double *part = local ? ctx.arena.get<double>(tag + "/rpart", (size_t)grid * K4) : nullptr;
+        size_t lds = local ? (size_t)K4 * 8 : 0;
+        hipLaunchKernelGGL(k_classify, dim3(grid), dim3(kClassifyThreads), lds, st, xyz, n, g, w,
+                           e, key0, reg0, counts, rstats, part, which, oor);
         MIA_HIP(hipGetLastError());
+        if (local) {
+            hipLaunchKernelGGL(k_reduce_rows, dim3(grid_for(K4, 128)), dim3(128), 0, st, part,
+                               (int)grid, K4, rstats);
+            MIA_HIP(hipGetLastError());
+        }
     }
     exclusive_scan<int32_t>(ctx, counts, out.start, keys, st, tag, 0);
     if (n > 0) {
         hipLaunchKernelGGL(k_scatter, dim3(grid_for(n, 256)), dim3(256), 0, st, xyz, n, g, w, axis,
-                           e, key0, reg0, out.start, cursor, out);
+                           e, key0, reg0, out.start, cursor, out, soa ? 1 : 0);
         MIA_HIP(hipGetLastError());
     }
 }
 
 void prepare_sample(Ctx &ctx, const mia_sample &s, Grid &g, Sorted &pos, Sorted &shp,
-                    double *region_stats, Sums &sums, hipStream_t st, const std::string &tag) {
-    unsigned long long *wmax = ctx.arena.get<unsigned long long>(tag + "/wmax", 2);
-    int *oor = ctx.arena.get<int>(tag + "/oor", 1);
-    fill<unsigned long long>(wmax, 0ull, 2, st);
-    fill<int32_t>(oor, 0, 1, st);
+                    double *region_stats, int *oor, bool soa, hipStream_t st,
+                    const std::string &tag) {
     if (region_stats && g.jk > 0) fill<double>(region_stats, 0.0, (int64_t)g.jk * g.jk * g.jk * 4, st);
     double *rs = g.jk > 0 ? region_stats : nullptr;
-    sort_one(ctx, s.pos, s.n_pos, s.pos_w, nullptr, nullptr, g, pos, rs, 0, wmax, oor, st,
+    sort_one(ctx, s.pos, s.n_pos, s.pos_w, nullptr, nullptr, g, pos, rs, 0, oor, soa, st,
              tag + "/pos");
-    sort_one(ctx, s.shp, s.n_shape, s.shp_w, s.axis, s.e, g, shp, rs, 1, wmax + 1, oor, st,
+    sort_one(ctx, s.shp, s.n_shape, s.shp_w, s.axis, s.e, g, shp, rs, 1, oor, soa, st,
              tag + "/shp");
-    unsigned long long hw[2];
-    int hoor = 0;
-    MIA_HIP(hipMemcpyAsync(hw, wmax, sizeof(hw), hipMemcpyDeviceToHost, st));
-    MIA_HIP(hipMemcpyAsync(&hoor, oor, sizeof(int), hipMemcpyDeviceToHost, st));
+}
+
+void check_prepared(const int *oor_dev, int n, hipStream_t st) {
+    std::vector<int> h(n > 0 ? n : 1, 0);
+    MIA_HIP(hipMemcpyAsync(h.data(), oor_dev, sizeof(int) * n, hipMemcpyDeviceToHost, st));
     MIA_HIP(hipStreamSynchronize(st));
-    double dw[2];
-    std::memcpy(dw, hw, sizeof(dw));
-    sums.wmax_pos = dw[0];
-    sums.wmax_shape = dw[1];
-    sums.out_of_range = hoor;
-    if (hoor)
-        throw Error{MIA_E_RANGE,
-                    "coordinates outside [0, boxsize] (or NaN) in a periodic sample"};
+    for (int i = 0; i < n; ++i)
+        if (h[i])
+            throw Error{MIA_E_RANGE, "coordinates outside [0, boxsize] (or NaN) in periodic sample " +
+                                         std::to_string(i)};
 }
 
 }  // namespace mia
