@@ -128,7 +128,7 @@ typedef struct mia_tuning {
     int32_t item_positions;     /* max positions per work item                        */
     int32_t waves;              /* persistent waves of the pair kernel                */
     int32_t hist_copies;        /* interleaved LDS copies of each histogram bin (1,2,4) */
-    int32_t reserved;
+    int32_t cell_shapes;        /* target mean shapes per cell for the automatic grid  */
 } mia_tuning;
 
 /* Per-call statistics, filled when the pointer is non-NULL. */

@@ -10,7 +10,8 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libmeasure_ia_amd.so")
+# MEASURE_IA_LIB overrides the library path (e.g. an alternative build).
+LIB_PATH = os.environ.get("MEASURE_IA_LIB") or os.path.join(HERE, "libmeasure_ia_amd.so")
 
 MIA_RPPI, MIA_RMU = 0, 1
 MIA_PLAIN, MIA_FAST = 0, 1
@@ -46,7 +47,7 @@ class MiaTuning(ctypes.Structure):
     _fields_ = [("columns_per_side", ctypes.c_int32), ("slabs_per_side", ctypes.c_int32),
                 ("slabs_per_block", ctypes.c_int32), ("block_shapes", ctypes.c_int32),
                 ("item_positions", ctypes.c_int32), ("waves", ctypes.c_int32),
-                ("hist_copies", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("hist_copies", ctypes.c_int32), ("cell_shapes", ctypes.c_int32)]
 
 
 class MiaStats(ctypes.Structure):

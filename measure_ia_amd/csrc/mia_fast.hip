@@ -339,16 +339,18 @@ struct PairArgs {
     const SampleDev *samples;
     int hist_stride;             // bytes per wave histogram
     int copies_log2;             // each bin is kept in 2^copies_log2 interleaved copies
-    int off_sp, off_dd, off_var; // byte offsets inside a wave histogram
+    int off_sp, off_sx, off_dd, off_var; // byte offsets inside a wave histogram
     unsigned long long *rechecks;
 };
 
 // Wave histogram layout (C = 2^copies_log2 interleaved copies of B bins;
-// slot h = bin * C + copy):
-//   u32     cnt[B*C]                at 0
-//   double2 {S+D, SxD}[B*C]         at off_sp   (one address, offsets 0 / 8)
-//   double  dd[B*C]                 at off_dd   (weighted samples)
-//   double  var[B*C]                at off_var  (variance requested)
+// slot h = bin * C + copy).  Separate arrays keep the FP64 slots 8 bytes
+// apart, so the lanes of one ds_add_f64 spread over all 16 bank pairs.
+//   u32     cnt[B*C]   at 0
+//   double  sp[B*C]    at off_sp    (S+D)
+//   double  sx[B*C]    at off_sx    (SxD)
+//   double  dd[B*C]    at off_dd    (weighted samples)
+//   double  var[B*C]   at off_var   (variance requested)
 
 // Shape record {a, b, l, ax, ay, e*w, w, 2*e*w} read through the scalar cache:
 // the inner loop is uniform over shapes, so every field lands in SGPRs.
@@ -357,7 +359,8 @@ typedef const __attribute__((address_space(4))) double *ShapePtr;
 __device__ void flush_hist(const PairArgs &P, char *hist, const SampleDev &S, int rs, int rp,
                            int lane) {
     uint32_t *cnt = (uint32_t *)hist;
-    double *spx = (double *)(hist + P.off_sp);
+    double *sp = (double *)(hist + P.off_sp);
+    double *sx = (double *)(hist + P.off_sx);
     double *dd = P.weighted ? (double *)(hist + P.off_dd) : nullptr;
     double *vr = P.var ? (double *)(hist + P.off_var) : nullptr;
     const int B = P.nbins;
@@ -368,11 +371,11 @@ __device__ void flush_hist(const PairArgs &P, char *hist, const SampleDev &S, in
         for (int k = 0; k < C; ++k) {
             int h = b * C + k;
             c += cnt[h];
-            vsp += spx[2 * h];
-            vsx += spx[2 * h + 1];
+            vsp += sp[h];
+            vsx += sx[h];
             cnt[h] = 0;
-            spx[2 * h] = 0.0;
-            spx[2 * h + 1] = 0.0;
+            sp[h] = 0.0;
+            sx[h] = 0.0;
             if (dd) {
                 vdd += dd[h];
                 dd[h] = 0.0;
@@ -449,118 +452,159 @@ __device__ __forceinline__ SampleView load_view(const SampleDev *sd) {
 // axis, or the reference's per-pair minimum-image wrap.
 enum { SH_NONE = 0, SH_CONST = 1, SH_WRAP = 2 };
 
+// A lane's pending histogram slot: consecutive pairs of one lane (one
+// position against the shapes of a block) that land in the same slot are
+// summed in registers and reach the LDS histogram as one atomic per run,
+// which cuts the same-address collisions of the hot (outer r) bins.
+struct Pend {
+    int b;
+    uint32_t c;
+    double sp, sx, dd, vr;
+};
+
+template <bool WEIGHTED>
+__device__ __forceinline__ void hist_add(const PairArgs &P, char *hist, int b, uint32_t c, double sp,
+                                         double sx, double dd, double vr) {
+    uint32_t *cnt = (uint32_t *)hist;
+    atomicAdd(&cnt[b], c);
+    double *hsp = (double *)(hist + P.off_sp);
+    double *hsx = (double *)(hist + P.off_sx);
+    atomicAdd(&hsp[b], sp);
+    atomicAdd(&hsx[b], sx);
+    if (WEIGHTED) atomicAdd(&((double *)(hist + P.off_dd))[b], dd);
+    if (P.var) atomicAdd(&((double *)(hist + P.off_var))[b], vr);
+}
+
+template <int MODE, int SH, bool WEIGHTED>
+__device__ __forceinline__ void one_pair(const PairArgs &P, const SampleView &S, bool valid,
+                                         double pa, double pb, double pl, double pw, double qa,
+                                         double qb, double ql, double qax, double qay, double qew,
+                                         double qw, double q2ew, double sha, double shb, double shl,
+                                         char *hist, const double *T, const double *X, int copy,
+                                         uint32_t &nre, Pend &pd) {
+    const int n_r = P.n_r, n_2 = P.n_2;
+    // the reference's own FP64 separation (measure_w_box_jk.py:159-162)
+    double da = qa - pa, db = qb - pb, dl = ql - pl;
+    if (SH == SH_WRAP) {
+        da = wrap_sep(da, S.L, S.half);
+        db = wrap_sep(db, S.L, S.half);
+        dl = wrap_sep(dl, S.L, S.half);
+    } else if (SH == SH_CONST) {
+        da = da + sha;
+        db = db + shb;
+        dl = dl + shl;
+    }
+    double a2 = da * da, b2 = db * db;
+    double s = a2 + b2;
+    double rv;
+    bool in;
+    if (MODE == MIA_RPPI) {
+        rv = s;
+        in = valid & (s >= P.T0) & (s < P.Tn) & (dl >= P.X0) & (dl < P.Xn);
+    } else {
+        rv = r2_column_order(a2, b2, dl * dl, S.los);
+        in = valid & (s > 0.0) & (rv >= P.T0) & (rv < P.Tn);
+    }
+    if (in) {
+        // bins: FP32 guess; exact FP64 threshold count when the guess lies
+        // within the guard band of a bin edge (every edge, the domain edges
+        // included, so no clamping is needed)
+        float rv32 = (float)rv;
+        float dlf = (float)dl;
+        float gr = __fmaf_rn(__log2f(rv32), P.lgs, P.lgc);
+        float g2;
+        if (MODE == MIA_RPPI)
+            g2 = __fmaf_rn(dlf, P.xs, P.xc);
+        else
+            g2 = __fmaf_rn(dlf, __builtin_amdgcn_rsqf(rv32), 1.0f) * P.xs;
+        float fr = floorf(gr), f2 = floorf(g2);
+        int ir = (int)fr, i2 = (int)f2;
+        bool near = (fabsf(gr - fr - 0.5f) > P.hr) | (fabsf(g2 - f2 - 0.5f) > P.h2);
+        if (near) {
+            ir = count_edges(T, n_r, rv);
+            if (MODE == MIA_RPPI) {
+                i2 = count_edges(X, n_2, dl);
+            } else {
+                double mu64 = dl / sqrt(rv);
+                i2 = mu64 < X[0] ? n_2 - 1 : count_edges(X, n_2, mu64);
+            }
+            ++nre;
+        }
+        // e+ = e cos 2phi = e (2 cos^2 phi - 1), ex = e sin 2phi = 2 e cos phi |sin phi|
+        // (|.| because phi = arccos(d) lies in [0, pi]); cos phi = t/r_p,
+        // |sin phi| = |c|/r_p with t, c the dot and cross products.
+        double t = fma(da, qax, db * qay);
+        double c = fma(da, qay, -(db * qax));
+        double r0 = __builtin_amdgcn_rcp(s);
+        double inv = fma(r0, fma(-s, r0, 1.0), r0);
+        double u = q2ew * t;
+        double ep = fma(u * t, inv, -qew);
+        double ex = (u * fabs(c)) * inv;
+        if (c * c < s * kSin2Exact) {
+            // nearly parallel: the reference's d may round outside [-1, 1],
+            // where arccos gives NaN and the pair adds no e+ (measure_w_box_jk.py:170-175)
+            double rp = sqrt(s);
+            double d = 0.0 + (da / rp) * qax;
+            d = d + (db / rp) * qay;
+            if (!(fabs(d) <= 1.0)) {
+                ep = 0.0;
+                ex = 0.0;
+            }
+        }
+        if (WEIGHTED) {
+            ep = pw * ep;
+            ex = pw * ex;
+        }
+        int b = ((__mul24(ir, n_2) + i2) << P.copies_log2) | copy;
+        double dd = WEIGHTED ? pw * qw : 0.0;
+        double vr = ep * ep;
+        if (b != pd.b) {
+            if (pd.b >= 0) hist_add<WEIGHTED>(P, hist, pd.b, pd.c, pd.sp, pd.sx, pd.dd, pd.vr);
+            pd.b = b;
+            pd.c = 0;
+            pd.sp = 0.0;
+            pd.sx = 0.0;
+            pd.dd = 0.0;
+            pd.vr = 0.0;
+        }
+        pd.c += 1;
+        pd.sp += ep;
+        pd.sx += ex;
+        if (WEIGHTED) pd.dd += dd;
+        if (P.var) pd.vr += vr;
+    }
+}
+
 template <int MODE, int SH, bool WEIGHTED>
 __device__ __forceinline__ void run_item(const PairArgs &P, const SampleView &S, int p0, int np,
                                          int shifts, ShapePtr shp, int ns, char *hist,
                                          const double *T, const double *X, int lane,
                                          uint32_t &nre) {
-    uint32_t *cnt = (uint32_t *)hist;
-    double *hspx = (double *)(hist + P.off_sp);
-    double *hdd = (double *)(hist + P.off_dd);
-    double *hvr = (double *)(hist + P.off_var);
-    const double L = S.L, half = S.half;
+    const double L = S.L;
     const double sha = SH == SH_CONST ? shift_val(shifts & 3, L) : 0.0;
     const double shb = SH == SH_CONST ? shift_val((shifts >> 2) & 3, L) : 0.0;
     const double shl = SH == SH_CONST ? shift_val((shifts >> 4) & 3, L) : 0.0;
-    const int n_r = P.n_r, n_2 = P.n_2;
-    const int cl2 = P.copies_log2;
-    const int copy = lane & ((1 << cl2) - 1);
+    const int copy = lane & ((1 << P.copies_log2) - 1);
     // Strided lane -> position map: lane l of batch b takes position b + l*nb,
     // so one batch spans the whole line-of-sight range of the item and its
     // pairs spread over more (r, pi) bins (fewer same-address LDS atomics).
     const int nb = (np + kWave - 1) / kWave;
+    Pend pd{-1, 0, 0, 0, 0, 0};
     for (int bt = 0; bt < nb; ++bt) {
-        int pi = bt + lane * nb;
-        bool valid = pi < np;
-        int p = p0 + (valid ? pi : 0);
-        double pa = S.pa[p], pb = S.pb[p], pl = S.pl[p];
-        double pw = WEIGHTED ? S.pw[p] : 1.0;
+        int pi0 = bt + lane * nb;
+        bool v0 = pi0 < np;
+        int p0i = p0 + (v0 ? pi0 : 0);
+        double pa0 = S.pa[p0i], pb0 = S.pb[p0i], pl0 = S.pl[p0i];
+        double pw0 = WEIGHTED ? S.pw[p0i] : 1.0;
         double qa = shp[0], qb = shp[1], ql = shp[2], qax = shp[3], qay = shp[4], qew = shp[5],
                qw = shp[6], q2ew = shp[7];
         for (int j = 0; j < ns; ++j) {
-            // prefetch the next shape record (one scalar load) while this one is used
+            // prefetch the next shape record (scalar loads) while this one is used
             ShapePtr rn = shp + 8 * (j + 1 < ns ? j + 1 : j);
             const double na = rn[0], nbv = rn[1], nl = rn[2], nax = rn[3], nay = rn[4],
                          new_ = rn[5], nw = rn[6], n2ew = rn[7];
-            // the reference's own FP64 separation (measure_w_box_jk.py:159-162)
-            double da = qa - pa, db = qb - pb, dl = ql - pl;
-            if (SH == SH_WRAP) {
-                da = wrap_sep(da, L, half);
-                db = wrap_sep(db, L, half);
-                dl = wrap_sep(dl, L, half);
-            } else if (SH == SH_CONST) {
-                da = da + sha;
-                db = db + shb;
-                dl = dl + shl;
-            }
-            double a2 = da * da, b2 = db * db;
-            double s = a2 + b2;
-            double rv;
-            bool in;
-            if (MODE == MIA_RPPI) {
-                rv = s;
-                in = valid & (s >= P.T0) & (s < P.Tn) & (dl >= P.X0) & (dl < P.Xn);
-            } else {
-                rv = r2_column_order(a2, b2, dl * dl, S.los);
-                in = valid & (s > 0.0) & (rv >= P.T0) & (rv < P.Tn);
-            }
-            if (in) {
-                // bins: FP32 guess; exact FP64 threshold count when the guess
-                // lies within the guard band of a bin edge (every edge, the
-                // domain edges included, so no clamping is needed)
-                float rv32 = (float)rv;
-                float dlf = (float)dl;
-                float gr = __fmaf_rn(__log2f(rv32), P.lgs, P.lgc);
-                float g2;
-                if (MODE == MIA_RPPI)
-                    g2 = __fmaf_rn(dlf, P.xs, P.xc);
-                else
-                    g2 = __fmaf_rn(dlf, __builtin_amdgcn_rsqf(rv32), 1.0f) * P.xs;
-                float fr = floorf(gr), f2 = floorf(g2);
-                int ir = (int)fr, i2 = (int)f2;
-                bool near = (fabsf(gr - fr - 0.5f) > P.hr) | (fabsf(g2 - f2 - 0.5f) > P.h2);
-                if (near) {
-                    ir = count_edges(T, n_r, rv);
-                    if (MODE == MIA_RPPI) {
-                        i2 = count_edges(X, n_2, dl);
-                    } else {
-                        double mu64 = dl / sqrt(rv);
-                        i2 = mu64 < X[0] ? n_2 - 1 : count_edges(X, n_2, mu64);
-                    }
-                    ++nre;
-                }
-                // e+ = e cos 2phi = e (2 cos^2 phi - 1), ex = e sin 2phi = 2 e cos phi |sin phi|
-                // (|.| because phi = arccos(d) lies in [0, pi]); cos phi = t/r_p,
-                // |sin phi| = |c|/r_p with t, c the dot and cross products.
-                double t = fma(da, qax, db * qay);
-                double c = fma(da, qay, -(db * qax));
-                double r0 = __builtin_amdgcn_rcp(s);
-                double inv = fma(r0, fma(-s, r0, 1.0), r0);
-                double u = q2ew * t;
-                double ep = fma(u * t, inv, -qew);
-                double ex = (u * fabs(c)) * inv;
-                if (c * c < s * kSin2Exact) {
-                    // nearly parallel: the reference's d may round outside [-1, 1],
-                    // where arccos gives NaN and the pair adds no e+ (measure_w_box_jk.py:170-175)
-                    double rp = sqrt(s);
-                    double d = 0.0 + (da / rp) * qax;
-                    d = d + (db / rp) * qay;
-                    if (!(fabs(d) <= 1.0)) {
-                        ep = 0.0;
-                        ex = 0.0;
-                    }
-                }
-                if (WEIGHTED) {
-                    ep = pw * ep;
-                    ex = pw * ex;
-                }
-                int b = ((__mul24(ir, n_2) + i2) << cl2) | copy;
-                atomicAdd(&cnt[b], 1u);
-                atomicAdd(&hspx[2 * b], ep);
-                atomicAdd(&hspx[2 * b + 1], ex);
-                if (WEIGHTED) atomicAdd(&hdd[b], pw * qw);
-                if (P.var) atomicAdd(&hvr[b], ep * ep);
-            }
+            one_pair<MODE, SH, WEIGHTED>(P, S, v0, pa0, pb0, pl0, pw0, qa, qb, ql, qax, qay, qew, qw,
+                                         q2ew, sha, shb, shl, hist, T, X, copy, nre, pd);
             qa = na;
             qb = nbv;
             ql = nl;
@@ -571,6 +615,7 @@ __device__ __forceinline__ void run_item(const PairArgs &P, const SampleView &S,
             q2ew = n2ew;
         }
     }
+    if (pd.b >= 0) hist_add<WEIGHTED>(P, hist, pd.b, pd.c, pd.sp, pd.sx, pd.dd, pd.vr);
 }
 
 template <int MODE, bool WEIGHTED>
@@ -885,11 +930,13 @@ void count_fast(Ctx &ctx, std::vector<FastSample> &samples, const mia_bins &bins
     A.hr = exact_only ? -1.0f : (float)(0.5 - dr);
     A.h2 = exact_only ? -1.0f : (float)(0.5 - d2);
     A.samples = sdd;
-    // histogram copies: enough LDS for ~5 waves per SIMD, at most 4 copies
+    // histogram copies (default 1: the pending-slot registers already absorb
+    // most collisions, and a single copy leaves LDS for 8 waves per SIMD);
+    // more copies only while LDS still holds ~5 waves per SIMD, at most 4
     int copies_log2 = 0;
     {
         size_t per_copy = (size_t)B * (4 + 16 + (any_w ? 8 : 0) + (any_var ? 8 : 0));
-        int want = tuning && tuning->hist_copies > 0 ? tuning->hist_copies : 2;
+        int want = tuning && tuning->hist_copies > 0 ? tuning->hist_copies : 1;
         while ((1 << (copies_log2 + 1)) <= want && (1 << (copies_log2 + 1)) <= 4 &&
                per_copy * (2u << copies_log2) * 20 <= 160 * 1024)
             ++copies_log2;
@@ -899,7 +946,9 @@ void count_fast(Ctx &ctx, std::vector<FastSample> &samples, const mia_bins &bins
     int off = 4 * BC;
     off = (off + 15) & ~15;
     A.off_sp = off;
-    off += 16 * BC;
+    off += 8 * BC;
+    A.off_sx = off;
+    off += 8 * BC;
     A.off_dd = off;
     if (A.weighted) off += 8 * BC;
     A.off_var = off;

@@ -286,10 +286,17 @@ Grid make_grid(const mia_sample &s, const mia_bins &bins, const mia_tuning *t) {
         reach_z = reach_r;
     int jk_eff = g.jk > 0 ? g.jk : 1;
     double span = g.L / jk_eff;
-    // Default: columns ~ reach/4 wide, slabs ~ reach_z/8 high (>= reach/4),
-    // capped so the key space stays small.
-    double cw_t = reach_r / 4.0;
-    double hz_t = std::fmax(reach_z / 8.0, reach_r / 4.0);
+    // Default cell: a cube holding ~cell_shapes shapes on average (cells
+    // that are too coarse waste candidate pairs, cells that are too fine
+    // leave the shape blocks too small to reuse each position load), kept
+    // within [reach/10, reach/2] across and [reach_z/10, reach_z/2] along
+    // the line of sight.  The default of 8 shapes per cell is the measured
+    // optimum of the 1M-position w and 2M-position multipole workloads.
+    double target = t && t->cell_shapes > 0 ? (double)t->cell_shapes : 8.0;
+    double nobj = (double)(s.n_shape > 0 ? s.n_shape : std::max<int64_t>(s.n_pos, 1));
+    double a = std::cbrt(target * g.L * g.L * g.L / nobj);
+    double cw_t = std::min(std::max(a, reach_r / 10.0), reach_r / 2.0);
+    double hz_t = std::min(std::max(a, reach_z / 10.0), std::max(reach_z / 2.0, reach_r / 2.0));
     int ncj = t && t->columns_per_side > 0 ? std::max(1, t->columns_per_side / jk_eff)
                                            : std::max(1, (int)std::lround(span / cw_t));
     int nzj = t && t->slabs_per_side > 0 ? std::max(1, t->slabs_per_side / jk_eff)
