@@ -1,20 +1,33 @@
 """Benchmark of the MI355X pair-counting engine (one JSON line on stdout).
 
-Workload (default "clustered1M"): 1,000,000 positions in a periodic box of
-L = 205 Mpc/h drawn as 5,000 Gaussian clusters (sigma = 1, Poisson(150)
-members) plus a uniform background; a random 200,000-galaxy shape subset with
-ellipticity aligned to the cluster centre (A = 0.1, noise N(0, 0.2));
-w_gg + w_g+ binning: 10 log r_p bins over 0.1-20, 20 linear pi bins over
-|pi| < 60, 27 jackknife regions.  Synthetic data, fixed seed.
+Workloads (synthetic, fixed seeds, built by measure_ia_amd/synth.py):
 
-A step is one full jackknife count of the sample (region ids, cell sort,
-work plan, pair kernel) with the inputs already in HBM, plus the copy of the
-histograms to the host.  ``value`` = in-domain pairs / second over the timed
+  clustered1M (default)  1,000,000 positions in a periodic box of L = 205 Mpc/h
+      drawn as 5,000 Gaussian clusters (sigma = 1, Poisson(150) members) plus a
+      uniform background; a random 200,000-galaxy shape subset with ellipticity
+      aligned to the cluster centre (A = 0.1, noise N(0, 0.2)); w_gg + w_g+
+      binning: 10 log r_p bins over 0.1-20, 20 linear pi bins over |pi| < 60,
+      27 jackknife regions.
+  uniform20k     20,000 uniform galaxies (shapes = positions), same binning.
+  multipoles2M   2,000,000 positions / 500,000 shapes as above with 10,000
+      clusters; (r, mu) binning: 15 log r bins over 0.1-60, 20 mu bins, 64
+      jackknife regions.
+  batch12        12 samples cut from one 3,000,000-galaxy clustered box, sizes
+      log-uniform in 10,000-800,000, shapes = 30 % of each, weights
+      U(0.5, 1.5); w_gg + w_g+ binning as clustered1M, 27 jackknife regions
+      each; all samples counted by one pair-kernel launch (mia_count_batch).
+  large20M       L = 500, 20,000,000 positions (20,000 clusters, sigma = 1.5,
+      30 % background) / 5,000,000 shapes (A = 0.05, noise N(0, 0.25));
+      15 log r_p bins over 0.1-100, 40 pi bins over |pi| < 100, 125 regions.
+
+A step is one full jackknife count of the workload (region ids, cell sort,
+work plan, pair kernel, histogram readback) with the inputs already in HBM.
+``value`` = in-domain (shape, position) pairs per second over the timed
 steps.  The pair kernel's own time comes from HIP events the library records
 on the launch stream.  Beside it: the plain FP64 GPU path (one thread per
-shape, global atomics) timed on a seeded 2% subset of the shapes, the
-end-to-end measure_xi_w call (host prep + count + estimators + HDF5), and the
-CPU baseline: the oracle's restatement of the reference's KDTree +
+shape, global atomics) timed on a seeded 2% shape subset, the end-to-end
+measure_xi_w call (host prep + count + estimators + HDF5), and the CPU
+baseline: the oracle's restatement of the reference's KDTree +
 multiprocessing counter timed on a seeded sample of shape chunks.
 """
 import argparse
@@ -33,15 +46,24 @@ PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 vector (= FP32 without packed instruc
 PEAK_FP32_PACKED_TFLOPS = 157.3
 FLOP_PER_PAIR = {"rppi": 28, "rmu": 32}   # algorithmic FLOP per in-domain pair
 
+W_BINS = dict(mode="rppi", r=(0.1, 20.0), nr=10, n2=20, pi_max=60.0, jk=3)
 WORKLOADS = {
-    "clustered1M": dict(cat="clustered1M", mode="rppi", r=(0.1, 20.0), nr=10, n2=20, pi_max=60.0, jk=3,
+    "clustered1M": dict(W_BINS, kind="single", cat="clustered1M",
                         desc="1,000,000 clustered positions / 200,000 aligned shapes, L=205, 27 JK, "
                              "10 log r_p bins 0.1-20, 20 pi bins |pi|<60"),
-    "uniform20k": dict(cat="uniform20k", mode="rppi", r=(0.1, 20.0), nr=10, n2=20, pi_max=60.0, jk=3,
+    "uniform20k": dict(W_BINS, kind="single", cat="uniform20k",
                        desc="20,000 uniform galaxies (shapes = positions), L=205, 27 JK"),
-    "multipoles2M": dict(cat="multipoles2M", mode="rmu", r=(0.1, 60.0), nr=15, n2=20, pi_max=None, jk=4,
+    "multipoles2M": dict(kind="single", cat="multipoles2M", mode="rmu", r=(0.1, 60.0), nr=15, n2=20,
+                         pi_max=None, jk=4,
                          desc="2,000,000 clustered positions / 500,000 shapes, L=205, 64 JK, "
                               "15 log r bins 0.1-60, 20 mu bins"),
+    "batch12": dict(W_BINS, kind="batch",
+                    desc="12 weighted samples (10k-800k, shapes 30%) from a 3,000,000-galaxy clustered "
+                         "box, L=205, 27 JK each, one batched launch"),
+    "large20M": dict(kind="single", cat="large20M", mode="rppi", r=(0.1, 100.0), nr=15, n2=40,
+                     pi_max=100.0, jk=5,
+                     desc="20,000,000 clustered positions / 5,000,000 shapes, L=500, 125 JK, "
+                          "15 log r_p bins 0.1-100, 40 pi bins |pi|<100"),
 }
 
 
@@ -49,23 +71,25 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_inputs(w):
-    from measure_ia_amd import binning, hostprep, synth
-    from measure_ia_amd.box import MeasureIABox  # noqa: F401
-    cat = synth.config(w["cat"])
-    L = cat.boxsize
-    p = hostprep.prepare(cat.data)
+def binning_for(w, L):
+    from measure_ia_amd import binning
     r_bins = np.logspace(np.log10(w["r"][0]), np.log10(w["r"][1]), w["nr"] + 1)
-    pi_max = w["pi_max"] if w["pi_max"] is not None else L / 2.0
-    pi_bins = np.linspace(-pi_max, pi_max, w["n2"] + 1)
-    mu_bins = np.linspace(-1, 1, w["n2"] + 1)
     if w["mode"] == "rppi":
-        th = binning.rppi_thresholds(r_bins, pi_bins, *w["r"])
-        sb = pi_bins
+        pi_max = w["pi_max"] if w["pi_max"] is not None else L / 2.0
+        sb = np.linspace(-pi_max, pi_max, w["n2"] + 1)
+        th = binning.rppi_thresholds(r_bins, sb, *w["r"])
     else:
-        th = binning.rmu_thresholds(r_bins, mu_bins, *w["r"])
-        sb = mu_bins
-    return cat, p, th, r_bins, sb
+        sb = np.linspace(-1, 1, w["n2"] + 1)
+        th = binning.rmu_thresholds(r_bins, sb, *w["r"])
+    return th, r_bins, sb
+
+
+def build_catalogues(w):
+    from measure_ia_amd import synth
+    if w["kind"] == "batch":
+        return synth.uneven_samples(12, parent=3_000_000, L=205.0, lo=10_000, hi=800_000,
+                                    shape_frac=0.3, seed=3)
+    return [synth.config(w["cat"])]
 
 
 def measured_traffic(workload):
@@ -77,23 +101,31 @@ def measured_traffic(workload):
         return float(json.load(f)["bytes_per_launch"])
 
 
-def cpu_baseline(cat, p, w, r_bins, sb, budget_chunks=48):
-    """Oracle tree + multiprocessing counter on a seeded sample of 1000-shape chunks."""
+def cpu_baseline(cat, w, r_bins, sb, pairs_per_shape):
+    """Oracle tree + multiprocessing counter on a seeded sample of shape chunks.
+
+    Chunks hold the reference's default 1000 shapes; where one such chunk would
+    take a core much longer than ~20 s (dense multipole searches), chunks are
+    made smaller so that one round of chunks over the workers stays near 20 s.
+    """
     from oracle import prep as oprep
     from oracle import tree
     workers = min(16, tree.default_workers())
+    core_rate = 3e6  # in-domain pairs per second per core, measured for this restatement
+    chunk = int(np.clip(20.0 * core_rate / max(pairs_per_shape, 1.0), 10, 1000))
+    budget_chunks = 3 * workers if chunk == 1000 else workers
     pr = oprep.prepare(cat.data)
     L = cat.boxsize
     jp, js = oprep.jackknife_region_indices(pr["positions"], pr["shapes"], L, w["jk"])
-    n_chunks = (len(pr["shapes"]) + 999) // 1000
+    n_chunks = (len(pr["shapes"]) + chunk - 1) // chunk
     rng = np.random.default_rng(12345)
     pick = sorted(rng.choice(n_chunks, size=min(budget_chunks, n_chunks), replace=False).tolist())
     out = tree.count_pairs(pr, r_bins, sb, w["r"][0], w["r"][1], L, mode=w["mode"], jk_pos=jp,
-                           jk_shape=js, num_box=w["jk"] ** 3, chunk_size=1000, num_nodes=workers,
+                           jk_shape=js, num_box=w["jk"] ** 3, chunk_size=chunk, num_nodes=workers,
                            chunk_subset=pick)
     rate = out["npairs"] / out["seconds"]
     return dict(value=rate, unit="pairs/s", cores=workers, kind="port",
-                sample=f"{len(pick)} of {n_chunks} seeded 1000-shape chunks ({out['npairs']} pairs, "
+                sample=f"{len(pick)} of {n_chunks} seeded {chunk}-shape chunks ({out['npairs']} pairs, "
                        f"{out['seconds']:.1f} s), KDTree + numpy + Pool restatement of "
                        f"measure_w_box_jk.py:543-857",
                 seconds_full_estimate=out["seconds"] * n_chunks / len(pick))
@@ -127,46 +159,79 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes": n * 8, "copies": 3}))
         return
+    from measure_ia_amd import hostprep
     from measure_ia_amd.engine import SampleArrays, get_engine
 
     w = WORKLOADS[a.workload]
+    batch = w["kind"] == "batch"
+    jk = w["jk"]
     t0 = time.perf_counter()
-    cat, p, th, r_bins, sb = build_inputs(w)
-    log(f"inputs built in {time.perf_counter() - t0:.1f} s: {p.n_pos} positions, {p.n_shape} shapes")
+    cats = build_catalogues(w)
+    L = cats[0].boxsize
+    th, r_bins, sb = binning_for(w, L)
+    preps = [hostprep.prepare(c.data) for c in cats]
+    log(f"inputs built in {time.perf_counter() - t0:.1f} s: " +
+        ", ".join(f"{p.n_pos}/{p.n_shape}" for p in preps))
     eng = get_engine(0)
     dev = torch.device("cuda", 0)
-    L = cat.boxsize
-    jk = w["jk"]
-    # inputs resident in HBM before the timed region
     to = lambda x: torch.from_numpy(np.ascontiguousarray(x)).to(dev)  # noqa: E731
-    s_dev = SampleArrays(to(p.positions), to(p.shapes), to(p.axis), to(p.e), None, None, L, p.los, jk,
-                         (L / 2.0) * 2.0 / jk)
+
+    def dev_sample(p, sel=None):
+        weighted = not (np.all(p.weight == 1.0) and np.all(p.weight_shape == 1.0))
+        sh = slice(None) if sel is None else sel
+        return SampleArrays(to(p.positions), to(p.shapes[sh]), to(p.axis[sh]), to(p.e[sh]),
+                            to(p.weight) if weighted else None,
+                            to(p.weight_shape[sh]) if weighted else None,
+                            L, p.los, jk, (L / 2.0) * 2.0 / jk)
+
+    # inputs resident in HBM before the timed region
+    samples = [dev_sample(p) for p in preps]
     torch.cuda.synchronize()
+
+    def step():
+        if batch:
+            return eng.count_batch(samples, th)
+        return [eng.count(samples[0], th, method="fast")]
+
     for _ in range(a.warmup):
-        res = eng.count(s_dev, th, method="fast")
+        res = step()
     torch.cuda.synchronize()
     kstats = []
     t1 = time.perf_counter()
     for _ in range(a.steps):
-        res = eng.count(s_dev, th, method="fast")
-        kstats.append(res.stats)
+        res = step()
+        kstats.append(res[0].stats)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t1
     ms_step = elapsed / a.steps * 1e3
-    pairs = int(res.npairs[0].sum())
+    pairs = int(sum(r.npairs[0].sum() for r in res))
     value = pairs / (ms_step / 1e3)
     k_ms = float(np.mean([s["ms_pairs"] for s in kstats]))
     flops = pairs * FLOP_PER_PAIR[w["mode"]]
     achieved = flops / (k_ms / 1e3) / 1e12
     log(f"fast: {ms_step:.2f} ms/step, pair kernel {k_ms:.2f} ms, {pairs} pairs")
 
+    loop = None
+    if batch:
+        # the same samples counted one call at a time
+        torch.cuda.synchronize()
+        tl = time.perf_counter()
+        single = [eng.count(s, th, method="fast") for s in samples]
+        torch.cuda.synchronize()
+        loop_ms = (time.perf_counter() - tl) * 1e3
+        for r1, rb in zip(single, res):
+            assert np.array_equal(r1.npairs, rb.npairs), "batched and single counts differ"
+        loop = dict(ms=loop_ms, pairs_per_s=pairs / (loop_ms / 1e3))
+        log(f"one call per sample: {loop_ms:.2f} ms")
+
+    big = int(np.argmax([p.n_shape for p in preps]))
     plain = None
     if not a.no_plain:
         frac = 0.02
+        p = preps[big]
         ns = max(1, int(p.n_shape * frac))
         sel = np.sort(np.random.default_rng(7).choice(p.n_shape, ns, replace=False))
-        s_pl = SampleArrays(s_dev.positions, to(p.shapes[sel]), to(p.axis[sel]), to(p.e[sel]), None, None,
-                            L, p.los, jk, (L / 2.0) * 2.0 / jk)
+        s_pl = dev_sample(p, sel)
         eng.count(s_pl, th, method="plain")
         torch.cuda.synchronize()
         tp = time.perf_counter()
@@ -182,11 +247,11 @@ def main():
         log(f"plain: {plain_rate:.3e} pairs/s on {frac:.0%} of shapes")
 
     e2e = None
-    if not a.no_e2e and w["mode"] == "rppi":
+    if not a.no_e2e and not batch and w["mode"] == "rppi":
         from measure_ia_amd import MeasureIABox
         with tempfile.TemporaryDirectory() as td:
             fn = os.path.join(td, "bench.hdf5")
-            box = MeasureIABox(cat.data, fn, None, None, list(w["r"]), w["nr"], w["n2"], w["pi_max"],
+            box = MeasureIABox(cats[0].data, fn, None, None, list(w["r"]), w["nr"], w["n2"], w["pi_max"],
                                boxsize=L, verbose=False)
             box.measure_xi_w("warm", "both", jk ** 3, temp_file_path=None)
             te = time.perf_counter()
@@ -197,31 +262,35 @@ def main():
     cpu = None
     if not a.no_cpu:
         tc = time.perf_counter()
-        cpu = cpu_baseline(cat, p, w, r_bins, sb)
+        pps = res[big].npairs[0].sum() / max(preps[big].n_shape, 1)
+        cpu = cpu_baseline(cats[big], w, r_bins, sb, pps)
         log(f"cpu baseline: {cpu['value']:.3e} pairs/s on {cpu['cores']} cores "
             f"({time.perf_counter() - tc:.1f} s)")
 
+    st = res[0].stats
     line = {
         "metric": "pairs/s & s per wgg+wg+ (1M gal, 27 JK): batched-FP32 vs plain GPU",
         "value": value, "unit": "pairs/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"{a.workload}: {w['desc']}", "mode": w["mode"], "jackknife": jk ** 3,
-                   "positions": int(p.n_pos), "shapes": int(p.n_shape), "pairs": pairs},
+                   "samples": len(preps), "positions": int(sum(p.n_pos for p in preps)),
+                   "shapes": int(sum(p.n_shape for p in preps)), "pairs": pairs},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": measured_traffic(a.workload),
                      "kernel": "k_pairs", "kernel_ms": k_ms,
                      "frac_of_fp32_packed_peak": achieved / PEAK_FP32_PACKED_TFLOPS,
-                     "candidate_efficiency": pairs / max(1, res.stats["candidates"]),
+                     "candidate_efficiency": pairs / max(1, st["candidates"]),
                      "model": f"{FLOP_PER_PAIR[w['mode']]} FLOP per in-domain pair; FP64 vector peak "
                               f"(the kernel is vector-ALU + LDS-atomic bound, no matrix work)"},
         "cpu_baseline": cpu,
         "plain_gpu": plain,
+        "one_call_per_sample": loop,
         "speedup_vs_plain": (value / plain["pairs_per_s"]) if plain else None,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
         "s_per_wgg_wgplus": ms_step / 1e3,
         "end_to_end_measure_xi_w_s": e2e,
-        "engine": {k: v for k, v in res.stats.items()},
+        "engine": dict(st),
     }
     print(json.dumps(line))
 
