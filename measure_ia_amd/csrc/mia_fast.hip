@@ -84,7 +84,8 @@ struct PlanArgs {
     int32_t *key_cursor; // emit pass
     const int32_t *key_off;
     Item *items;
-    int64_t *cost;       // emit pass: ns * np of every item
+    int64_t *cost;       // emit pass: balancing cost of every item
+    unsigned long long *candidates;  // emit pass: sum of ns * np
     int emit;
     int local;           // 1: aggregate per workgroup in LDS (K1^2 keys fit)
 };
@@ -217,6 +218,7 @@ __device__ __forceinline__ void plan_ranges(const PlanArgs &P, int64_t t, F &&em
 }
 
 constexpr int kPlanThreads = 256;
+constexpr int64_t kItemOverhead = 256;  // per-item cost, in pair evaluations
 
 // Count (emit == 0) or emit the work items of every (block, stencil column)
 // pair, counting-sorted by key (sample, R_s, R_p).  With `local` the key
@@ -258,8 +260,12 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan_items(PlanArgs P) {
         }
         __syncthreads();
     }
-    if (!P.emit || !active) return;
-    plan_ranges(P, t, [&](int k, int pb, int pe, int rp, uint8_t sh, int blk, int ns) {
+    if (!P.emit) return;
+    __shared__ unsigned long long cand_l;
+    if (threadIdx.x == 0) cand_l = 0;
+    __syncthreads();
+    unsigned long long cand = 0;
+    if (active) plan_ranges(P, t, [&](int k, int pb, int pe, int rp, uint8_t sh, int blk, int ns) {
         int n = (pe - pb + pmax - 1) / pmax;
         int slot = P.key_off[kb + k] +
                    (local ? lkey[nk + k] + atomicAdd(&lkey[k], n) : atomicAdd(&P.key_cursor[kb + k], n));
@@ -273,9 +279,15 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan_items(PlanArgs P) {
             it.shifts = sh;
             it.pad = 0;
             P.items[slot + q] = it;
-            P.cost[slot + q] = (int64_t)ns * it.np;
+            // a wave pays for whole 64-lane position batches plus a fixed
+            // per-item overhead, whatever the item's true pair count
+            P.cost[slot + q] = (int64_t)ns * (((it.np + kWave - 1) / kWave) * kWave) + kItemOverhead;
+            cand += (unsigned long long)ns * it.np;
         }
     });
+    if (cand) atomicAdd(&cand_l, cand);
+    __syncthreads();
+    if (threadIdx.x == 0 && cand_l) atomicAdd(P.candidates, cand_l);
 }
 
 __global__ void k_blocks_count(const int32_t *sstart, Grid g, int ng, int G, int bsz,
@@ -811,6 +823,7 @@ void count_fast(Ctx &ctx, std::vector<FastSample> &samples, const mia_bins &bins
         P.key_off = ko;
         P.items = nullptr;
         P.cost = nullptr;
+        P.candidates = nullptr;
         P.emit = 0;
         P.local = plan_local;
         int64_t nt = (int64_t)P.nblocks * P.nxw * P.nxw;
@@ -828,11 +841,14 @@ void count_fast(Ctx &ctx, std::vector<FastSample> &samples, const mia_bins &bins
     Item *items = ctx.arena.get<Item>("fast/items", (size_t)std::max<int64_t>(nitems, 1));
     int64_t *cost = ctx.arena.get<int64_t>("fast/cost", nitems + 1);
     int64_t *cpre = ctx.arena.get<int64_t>("fast/costpre", nitems + 1);
+    unsigned long long *ncand = ctx.arena.get<unsigned long long>("fast/candidates", 1);
+    fill<unsigned long long>(ncand, 0ull, 1, st);
     for (int si = 0; si < ns_; ++si) {
         PlanArgs P = plans[si];
         P.emit = 1;
         P.items = items;
         P.cost = cost;
+        P.candidates = ncand;
         int64_t nt = (int64_t)P.nblocks * P.nxw * P.nxw;
         if (nt > 0) {
             hipLaunchKernelGGL(k_plan_items, dim3(nblk(nt, kPlanThreads)), dim3(kPlanThreads),
@@ -984,14 +1000,15 @@ void count_fast(Ctx &ctx, std::vector<FastSample> &samples, const mia_bins &bins
     MIA_HIP(hipEventElapsedTime(&t01, ev[0], ev[1]));
     MIA_HIP(hipEventElapsedTime(&t12, ev[1], ev[2]));
     MIA_HIP(hipEventElapsedTime(&t23, ev[2], ev[3]));
-    unsigned long long nre = 0;
+    unsigned long long nre = 0, ncand_h = 0;
     MIA_HIP(hipMemcpy(&nre, rechecks, sizeof(nre), hipMemcpyDeviceToHost));
+    MIA_HIP(hipMemcpy(&ncand_h, ncand, sizeof(ncand_h), hipMemcpyDeviceToHost));
     for (auto &e : ev) (void)hipEventDestroy(e);
     if (stats) {
         stats->ms_plan = t01;
         stats->ms_pairs = t12;
         stats->ms_finalize = t23;
-        stats->candidates = total_cost;
+        stats->candidates = (int64_t)ncand_h;
         stats->items = nitems;
         stats->rechecks = (int64_t)nre;
         stats->slabs_per_block = gsz.empty() ? 0 : gsz[0];
