@@ -40,7 +40,7 @@ enum mia_status {
     MIA_E_ARG = -1,      /* invalid argument or inconsistent sizes            */
     MIA_E_HIP = -2,      /* a HIP runtime call failed                          */
     MIA_E_NOMEM = -3,    /* device allocation failed                           */
-    MIA_E_RANGE = -4,    /* coordinates outside [0, boxsize] for a periodic box */
+    MIA_E_RANGE = -4,    /* coordinates outside [0, boxsize] (or NaN)            */
     MIA_E_UNSUPPORTED = -5
 };
 
@@ -75,7 +75,10 @@ typedef struct mia_sample {
     double boxsize;         /* L (periodic box [0, L)^3)                            */
     double jk_sub;          /* L_sub = 2*L_0p5/L_subboxes (measure_IA_base.py:428)  */
     int32_t los;            /* line-of-sight column, 0..2                           */
-    int32_t periodic;       /* must be 1 (minimum-image wrap, measure_w_box_jk.py:160-162) */
+    int32_t periodic;       /* 1: minimum-image wrap (measure_w_box_jk.py:160-162); 0: none,
+                               as MeasureIABox(periodicity=False).  Either way every
+                               coordinate must lie in [0, boxsize], as the reference's
+                               KDTree over the box requires (measure_w_box_jk.py:386). */
     int32_t jk_per_side;    /* L_subboxes; 0 = no jackknife                         */
     int32_t reserved;
 } mia_sample;

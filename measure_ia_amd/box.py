@@ -160,8 +160,6 @@ class MeasureIABox(SimInfo):
     def _count(self, mode, masks, L_subboxes, ellipticity, variance):
         """Prepare, count on the device and assemble every estimator grid."""
         from .engine import SampleArrays
-        if not self.periodicity:
-            raise NotImplementedError("the device counters need periodicity=True")
         t0 = time.perf_counter()
         p = hostprep.prepare(self.data, masks, ellipticity)
         if self.verbose:
@@ -172,7 +170,8 @@ class MeasureIABox(SimInfo):
         sample = SampleArrays(positions=p.positions, shapes=p.shapes, axis=p.axis, e=p.e,
                               weight=p.weight, weight_shape=p.weight_shape, boxsize=self.boxsize,
                               los=p.los, jk_per_side=jk,
-                              jk_sub=(self.L_0p5 * 2.0 / jk) if jk else 0.0, periodic=True)
+                              jk_sub=(self.L_0p5 * 2.0 / jk) if jk else 0.0,
+                              periodic=bool(self.periodicity))
         t1 = time.perf_counter()
         res = self._engine().count(sample, th, method=BACKENDS[self.backend], variance=variance,
                                    tuning=self.tuning)

@@ -27,8 +27,7 @@ void check_sample(const mia_sample *s) {
     MIA_REQUIRE(s->los >= 0 && s->los <= 2, "los must be 0, 1 or 2");
     MIA_REQUIRE(s->jk_per_side >= 0 && s->jk_per_side <= 64, "jk_per_side out of range");
     MIA_REQUIRE(s->jk_per_side == 0 || s->jk_sub > 0, "jk_sub must be positive with jackknife");
-    if (s->periodic != 1)
-        throw mia::Error{MIA_E_UNSUPPORTED, "only periodic boxes are supported by the device path"};
+    MIA_REQUIRE(s->periodic == 0 || s->periodic == 1, "periodic must be 0 or 1");
 }
 
 void check_bins(const mia_bins *b) {

@@ -184,7 +184,10 @@ __device__ __forceinline__ void plan_ranges(const PlanArgs &P, int64_t t, F &&em
     const int kr = B.rs * P.K1;
     auto range = [&](int p_begin, int p_end, int rp, int cz) {
         if (p_end <= p_begin) return;
-        bool wrap = (ca == 3) || (cb == 3) || (cz == 3);
+        // mixed shift classes take the per-pair minimum-image wrap; a
+        // non-periodic box never wraps (classes are always 0 there, and the
+        // anomalous tail's "any LOS value" class means no shift)
+        bool wrap = P.periodic && ((ca == 3) || (cb == 3) || (cz == 3));
         uint8_t sh = (uint8_t)((ca == 3 ? 0 : ca) | ((cb == 3 ? 0 : cb) << 2) |
                                ((cz == 3 ? 0 : cz) << 4) | (wrap ? 64 : 0));
         emit(kr + rp, p_begin, p_end, rp, sh, blk, B.ns);

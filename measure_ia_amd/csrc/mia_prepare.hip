@@ -388,7 +388,7 @@ void check_prepared(const int *oor_dev, int n, hipStream_t st) {
     MIA_HIP(hipStreamSynchronize(st));
     for (int i = 0; i < n; ++i)
         if (h[i])
-            throw Error{MIA_E_RANGE, "coordinates outside [0, boxsize] (or NaN) in periodic sample " +
+            throw Error{MIA_E_RANGE, "coordinates outside [0, boxsize] (or NaN) in sample " +
                                          std::to_string(i)};
 }
 

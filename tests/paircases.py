@@ -12,7 +12,8 @@ from oracle import brute, prep
 def make_case(name):
     """Return (data, boxsize, cfg) where cfg holds binning and jackknife settings."""
     rng = np.random.default_rng(zlib.crc32(name.encode()))
-    cfg = dict(r_min=0.1, r_max=10.0, nr=10, n2=8, pi_max=20.0, L_sub=3, los=2, weights=False)
+    cfg = dict(r_min=0.1, r_max=10.0, nr=10, n2=8, pi_max=20.0, L_sub=3, los=2, weights=False,
+               periodic=True)
     if name == "uniform_jk27":
         L = 60.0
         pos = wrap_box(rng.uniform(0, L, (3000, 3)), L)
@@ -47,6 +48,13 @@ def make_case(name):
         pos = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
         shp = pos[rng.choice(len(pos), 600, replace=False)]
         cfg.update(r_min=1.0, r_max=5.0, nr=6, n2=10, pi_max=5.0, L_sub=3)
+    elif name == "open_box_jk27":
+        # periodicity=False: no minimum-image wrap, pairs across the box faces
+        # fall outside the domain (measure_w_box_jk.py:160-162)
+        L = 30.0
+        pos = wrap_box(rng.uniform(0, L, (2500, 3)), L)
+        shp = wrap_box(rng.uniform(0, L, (1200, 3)), L)
+        cfg.update(periodic=False, r_max=12.0, pi_max=14.0)
     elif name == "tiny_box_wrap":
         L = 9.0
         pos = wrap_box(rng.uniform(0, L, (800, 3)), L)
@@ -66,7 +74,7 @@ def make_case(name):
 
 
 CASES = ["uniform_jk27", "uniform_auto_nojk", "clustered_jk8", "weighted_los0", "los1_fullpi",
-         "lattice_boundaries", "tiny_box_wrap"]
+         "lattice_boundaries", "tiny_box_wrap", "open_box_jk27"]
 
 
 def oracle_counts(data, L, cfg, mode):
@@ -79,7 +87,7 @@ def oracle_counts(data, L, cfg, mode):
     if K:
         jp, js = prep.jackknife_region_indices(pr["positions"], pr["shapes"], L, cfg["L_sub"])
     out = brute.count_pairs(pr, r_bins, sb, cfg["r_min"], cfg["r_max"], L, mode=mode,
-                            jk_pos=jp, jk_shape=js, num_box=K)
+                            jk_pos=jp, jk_shape=js, num_box=K, periodic=cfg["periodic"])
     return pr, out, (r_bins, pi_bins, mu_bins)
 
 
@@ -90,7 +98,7 @@ def engine_sample(data, L, cfg):
                            weight=p.weight if cfg["weights"] else None,
                            weight_shape=p.weight_shape if cfg["weights"] else None,
                            boxsize=L, los=p.los, jk_per_side=n,
-                           jk_sub=(L / 2.0) * 2.0 / n if n else 0.0)
+                           jk_sub=(L / 2.0) * 2.0 / n if n else 0.0, periodic=cfg["periodic"])
 
 
 def thresholds(cfg, L, mode):

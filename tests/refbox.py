@@ -14,7 +14,7 @@ from oracle import brute, post, prep
 
 
 def expected_output(data, boxsize, snap_group, r_lim, nr, n2, pi_max, grp, ds, corr_type, num_jk,
-                    layout, masks=None, ellipticity="distortion"):
+                    layout, masks=None, ellipticity="distortion", periodic=True):
     pr = prep.prepare(data, masks, ellipticity)
     r_bins, pi_bins, mu_bins = prep.bin_edges(r_lim[0], r_lim[1], nr, n2, pi_max, boxsize)
     mode = "rppi" if grp == "w" else "rmu"
@@ -25,7 +25,7 @@ def expected_output(data, boxsize, snap_group, r_lim, nr, n2, pi_max, grp, ds, c
     if num_jk:
         jp, js = prep.jackknife_region_indices(pr["positions"], pr["shapes"], boxsize, L_sub)
     c = brute.count_pairs(pr, r_bins, sb, r_lim[0], r_lim[1], boxsize, mode=mode, jk_pos=jp,
-                          jk_shape=js, num_box=num_jk)
+                          jk_shape=js, num_box=num_jk, periodic=periodic)
     if num_jk:
         jk = dict(jk_pos=jp, jk_shape=js, num_box=num_jk,
                   R_jk=prep.responsivity_jk(pr["weight_shape"], pr["e"], js, num_jk))

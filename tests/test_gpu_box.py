@@ -99,6 +99,22 @@ def test_masks_weights_los(tmp_path):
     _compare(_read_all(fn), exp, exact_gg=False)
 
 
+@pytest.mark.parametrize("grp", ["w", "m"])
+def test_open_box_matches_reference_layout(tmp_path, grp):
+    """MeasureIABox(periodicity=False): separations are not wrapped."""
+    d = _data(seed=13)
+    fn = str(tmp_path / "out.hdf5")
+    box = MeasureIABox(d, fn, None, None, [0.1, 10.0], 10, 8, 20.0, boxsize=L, periodicity=False,
+                       verbose=False)
+    if grp == "w":
+        box.measure_xi_w("open", "both", 8, temp_file_path=str(tmp_path))
+    else:
+        box.measure_xi_multipoles("open", "both", 8, temp_file_path=str(tmp_path))
+    exp = expected_output(d, L, "", [0.1, 10.0], 10, 8, 20.0, grp, "open", "both", 8, "tree",
+                          periodic=False)
+    _compare(_read_all(fn), exp)
+
+
 def test_weight_scaling_like_reference(tmp_path):
     """tests/test_weights.py:5-43 of the reference: weights 0.5 -> w and DD / 4, cov / 16."""
     d = _data(seed=7)

@@ -172,3 +172,28 @@ def test_tree_equals_brute(mode):
     np.testing.assert_allclose(a["SplusD"], b["SplusD"], rtol=1e-10, atol=1e-13)
     np.testing.assert_allclose(a["SplusD_jk"], b["SplusD_jk"], rtol=1e-10, atol=1e-13)
     assert a["npairs"] == b["npairs"]
+
+
+@pytest.mark.parametrize("mode", ["rppi", "rmu"])
+def test_open_box_scalar_loop_and_subset_of_periodic(mode):
+    """periodicity=False (measure_w_box_jk.py:160-162: no wrap).
+
+    The brute counter equals the scalar loop without wrap, and every pair it
+    keeps is also kept, in the same bin, by the periodic count (an unwrapped
+    in-domain separation is below L/2, where the wrap changes nothing), while
+    the periodic count finds more pairs across the faces.
+    """
+    L = 10.0
+    d = _sample(80, 50, L, seed=7, weights=True)
+    pr = prep.prepare(d)
+    r_bins, pi_bins, mu_bins = prep.bin_edges(0.2, 4.0, 5, 6, 3.0, L)
+    sb = pi_bins if mode == "rppi" else mu_bins
+    op = brute.count_pairs(pr, r_bins, sb, 0.2, 4.0, L, mode=mode, periodic=False)
+    sc = brute.count_pairs_scalar(pr, r_bins, sb, 0.2, 4.0, L, mode=mode, periodic=False)
+    got = np.zeros_like(op["DD"])
+    for (i, j), (dd, ep, ex) in sc.items():
+        got[i, j] = dd
+    np.testing.assert_allclose(op["DD"], got, rtol=1e-13)
+    per = brute.count_pairs(pr, r_bins, sb, 0.2, 4.0, L, mode=mode, periodic=True)
+    assert np.all(op["NP"] <= per["NP"])
+    assert op["NP"].sum() < per["NP"].sum()
