@@ -122,7 +122,9 @@ typedef struct mia_counts {
     double *region_stats;
 } mia_counts;
 
-/* Optional tuning (NULL = automatic).  Zero fields mean "choose". */
+/* Optional tuning (NULL = automatic).  Zero fields mean "choose": the grid
+ * (columns, slabs, slabs per block) then comes from a cost-model search over
+ * the sample's sizes, box and binning reach. */
 typedef struct mia_tuning {
     int32_t columns_per_side;   /* projected-plane cells per box side                 */
     int32_t slabs_per_side;     /* line-of-sight slabs per box side                   */
@@ -131,7 +133,8 @@ typedef struct mia_tuning {
     int32_t item_positions;     /* max positions per work item                        */
     int32_t waves;              /* persistent waves of the pair kernel                */
     int32_t hist_copies;        /* interleaved LDS copies of each histogram bin (1,2,4) */
-    int32_t cell_shapes;        /* target mean shapes per cell for the automatic grid  */
+    int32_t cell_shapes;        /* > 0: size cells to hold this many shapes on average
+                                   instead of the cost-model search                     */
 } mia_tuning;
 
 /* Per-call statistics, filled when the pointer is non-NULL. */

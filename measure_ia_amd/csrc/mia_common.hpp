@@ -84,6 +84,7 @@ struct Grid {
     int ncj = 1, nzj = 1;  // per jackknife sub-box (nc = jk*ncj when jk > 0)
     double cw = 0, hz = 0; // nominal column width and slab height
     double margin = 0;     // geometric slack for rounding at cell edges
+    int G = 0;             // slabs per shape block chosen with the grid (0: decide later)
     int keys() const { return nc * nc * (nz + 1); }
 };
 
@@ -139,7 +140,8 @@ template <class T>
 void exclusive_scan(Ctx &ctx, const T *in, T *out, int64_t n, hipStream_t st,
                     const std::string &tag, int depth);
 
-// Pick columns/slabs per side for a sample (autotuned defaults).
+// Pick columns/slabs per side and slabs per block for a sample: explicit
+// tuning fields win; otherwise a cost-model search (mia_grid.hip).
 Grid make_grid(const mia_sample &s, const mia_bins &bins, const mia_tuning *t);
 
 // ---------------------------------------------------------------------------

@@ -749,10 +749,13 @@ void count_fast(Ctx &ctx, std::vector<FastSample> &samples, const mia_bins &bins
         FastSample &F = samples[si];
         const Grid &g = F.g;
         K1max = std::max(K1max, g.jk > 0 ? g.jk * g.jk * g.jk : 1);
-        // slabs per block: aim for ~16 shapes per block, dividing the per-jk slab count
+        // slabs per block: explicit, else the grid's choice, else ~16 shapes
+        // per block; always a divisor of the per-jk slab count
         int G = 1;
         if (tuning && tuning->slabs_per_block > 0) {
             G = tuning->slabs_per_block;
+        } else if (g.G > 0) {
+            G = g.G;
         } else {
             double per_slab = (double)F.shp.n / ((double)g.nc * g.nc * g.nz);
             double want = per_slab > 0 ? 16.0 / per_slab : g.nzj;

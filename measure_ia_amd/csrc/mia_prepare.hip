@@ -265,54 +265,6 @@ template void exclusive_scan<int32_t>(Ctx &, const int32_t *, int32_t *, int64_t
 template void exclusive_scan<int64_t>(Ctx &, const int64_t *, int64_t *, int64_t, hipStream_t,
                                       const std::string &, int);
 
-Grid make_grid(const mia_sample &s, const mia_bins &bins, const mia_tuning *t) {
-    Grid g;
-    g.L = s.boxsize;
-    g.half = s.boxsize / 2.0;
-    g.los = s.los;
-    int k = 0;
-    int na[2];
-    for (int c = 0; c < 3; ++c)
-        if (c != s.los) na[k++] = c;
-    g.ax_a = na[0];
-    g.ax_b = na[1];
-    g.jk = s.jk_per_side > 0 ? s.jk_per_side : 0;
-    g.jk_sub = g.jk > 0 ? s.jk_sub : s.boxsize;
-    double reach_r = std::sqrt(bins.r2_edges[bins.n_r]);
-    double reach_z;
-    if (bins.mode == MIA_RPPI)
-        reach_z = std::fmax(std::fabs(bins.x2_edges[0]), std::fabs(bins.x2_edges[bins.n_2]));
-    else
-        reach_z = reach_r;
-    int jk_eff = g.jk > 0 ? g.jk : 1;
-    double span = g.L / jk_eff;
-    // Default cell: a cube holding ~cell_shapes shapes on average (cells
-    // that are too coarse waste candidate pairs, cells that are too fine
-    // leave the shape blocks too small to reuse each position load), kept
-    // within [reach/10, reach/2] across and [reach_z/10, reach_z/2] along
-    // the line of sight.  The default of 8 shapes per cell is the measured
-    // optimum of the 1M-position w and 2M-position multipole workloads.
-    double target = t && t->cell_shapes > 0 ? (double)t->cell_shapes : 8.0;
-    double nobj = (double)(s.n_shape > 0 ? s.n_shape : std::max<int64_t>(s.n_pos, 1));
-    double a = std::cbrt(target * g.L * g.L * g.L / nobj);
-    double cw_t = std::min(std::max(a, reach_r / 10.0), reach_r / 2.0);
-    double hz_t = std::min(std::max(a, reach_z / 10.0), std::max(reach_z / 2.0, reach_r / 2.0));
-    int ncj = t && t->columns_per_side > 0 ? std::max(1, t->columns_per_side / jk_eff)
-                                           : std::max(1, (int)std::lround(span / cw_t));
-    int nzj = t && t->slabs_per_side > 0 ? std::max(1, t->slabs_per_side / jk_eff)
-                                         : std::max(1, (int)std::lround(span / hz_t));
-    while (ncj * jk_eff > 512 && ncj > 1) --ncj;
-    while (nzj * jk_eff > 512 && nzj > 1) --nzj;
-    g.ncj = ncj;
-    g.nzj = nzj;
-    g.nc = ncj * jk_eff;
-    g.nz = nzj * jk_eff;
-    g.cw = g.L / g.nc;
-    g.hz = g.L / g.nz;
-    g.margin = 1e-9 * g.L + 1e-12;
-    return g;
-}
-
 void region_ids(Ctx &ctx, const double *xyz, int64_t n, double jk_sub, int jk, int32_t *region,
                 hipStream_t st) {
     (void)ctx;
