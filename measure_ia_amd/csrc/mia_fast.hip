@@ -350,7 +350,7 @@ struct PairArgs {
     double T0, Tn, X0, Xn;       // domain bounds (rppi: s and pi; rmu: r^2)
     float lgs, lgc;              // r bin guess: log2(v) * lgs + lgc
     float xs, xc;                // pi: pi * xs + xc; mu: (mu + 1) * xs
-    float hr, h2;                // exact path when |frac(guess) - 1/2| > h (h = 1/2 - guard band)
+    float dr, d2;                // guard bands: exact path when |guess - nearest integer| < d
     const SampleDev *samples;
     int hist_stride;             // bytes per wave histogram
     int copies_log2;             // each bin is kept in 2^copies_log2 interleaved copies
@@ -477,7 +477,7 @@ struct Pend {
     double sp, sx, dd, vr;
 };
 
-template <bool WEIGHTED>
+template <bool WEIGHTED, bool VAR>
 __device__ __forceinline__ void hist_add(const PairArgs &P, char *hist, int b, uint32_t c, double sp,
                                          double sx, double dd, double vr) {
     uint32_t *cnt = (uint32_t *)hist;
@@ -487,10 +487,10 @@ __device__ __forceinline__ void hist_add(const PairArgs &P, char *hist, int b, u
     atomicAdd(&hsp[b], sp);
     atomicAdd(&hsx[b], sx);
     if (WEIGHTED) atomicAdd(&((double *)(hist + P.off_dd))[b], dd);
-    if (P.var) atomicAdd(&((double *)(hist + P.off_var))[b], vr);
+    if (VAR) atomicAdd(&((double *)(hist + P.off_var))[b], vr);
 }
 
-template <int MODE, int SH, bool WEIGHTED>
+template <int MODE, int SH, bool WEIGHTED, bool VAR>
 __device__ __forceinline__ void one_pair(const PairArgs &P, const SampleView &S, bool valid,
                                          double pa, double pb, double pl, double pw, double qa,
                                          double qb, double ql, double qax, double qay, double qew,
@@ -532,9 +532,11 @@ __device__ __forceinline__ void one_pair(const PairArgs &P, const SampleView &S,
             g2 = __fmaf_rn(dlf, P.xs, P.xc);
         else
             g2 = __fmaf_rn(dlf, __builtin_amdgcn_rsqf(rv32), 1.0f) * P.xs;
-        float fr = floorf(gr), f2 = floorf(g2);
-        int ir = (int)fr, i2 = (int)f2;
-        bool near = (fabsf(gr - fr - 0.5f) > P.hr) | (fabsf(g2 - f2 - 0.5f) > P.h2);
+        // truncation is floor for the non-negative guesses; a guess within
+        // the guard band of an integer (an edge, or a slightly negative
+        // guess at the lower domain edge) takes the exact path
+        int ir = (int)gr, i2 = (int)g2;
+        bool near = (fabsf(gr - rintf(gr)) < P.dr) | (fabsf(g2 - rintf(g2)) < P.d2);
         if (near) {
             ir = count_edges(T, n_r, rv);
             if (MODE == MIA_RPPI) {
@@ -572,9 +574,9 @@ __device__ __forceinline__ void one_pair(const PairArgs &P, const SampleView &S,
         }
         int b = ((__mul24(ir, n_2) + i2) << P.copies_log2) | copy;
         double dd = WEIGHTED ? pw * qw : 0.0;
-        double vr = ep * ep;
+        double vr = VAR ? ep * ep : 0.0;
         if (b != pd.b) {
-            if (pd.b >= 0) hist_add<WEIGHTED>(P, hist, pd.b, pd.c, pd.sp, pd.sx, pd.dd, pd.vr);
+            if (pd.b >= 0) hist_add<WEIGHTED, VAR>(P, hist, pd.b, pd.c, pd.sp, pd.sx, pd.dd, pd.vr);
             pd.b = b;
             pd.c = 0;
             pd.sp = 0.0;
@@ -586,11 +588,11 @@ __device__ __forceinline__ void one_pair(const PairArgs &P, const SampleView &S,
         pd.sp += ep;
         pd.sx += ex;
         if (WEIGHTED) pd.dd += dd;
-        if (P.var) pd.vr += vr;
+        if (VAR) pd.vr += vr;
     }
 }
 
-template <int MODE, int SH, bool WEIGHTED>
+template <int MODE, int SH, bool WEIGHTED, bool VAR>
 __device__ __forceinline__ void run_item(const PairArgs &P, const SampleView &S, int p0, int np,
                                          int shifts, ShapePtr shp, int ns, char *hist,
                                          const double *T, const double *X, int lane,
@@ -618,7 +620,7 @@ __device__ __forceinline__ void run_item(const PairArgs &P, const SampleView &S,
             ShapePtr rn = shp + 8 * (j + 1 < ns ? j + 1 : j);
             const double na = rn[0], nbv = rn[1], nl = rn[2], nax = rn[3], nay = rn[4],
                          new_ = rn[5], nw = rn[6], n2ew = rn[7];
-            one_pair<MODE, SH, WEIGHTED>(P, S, v0, pa0, pb0, pl0, pw0, qa, qb, ql, qax, qay, qew, qw,
+            one_pair<MODE, SH, WEIGHTED, VAR>(P, S, v0, pa0, pb0, pl0, pw0, qa, qb, ql, qax, qay, qew, qw,
                                          q2ew, sha, shb, shl, hist, T, X, copy, nre, pd);
             qa = na;
             qb = nbv;
@@ -630,10 +632,10 @@ __device__ __forceinline__ void run_item(const PairArgs &P, const SampleView &S,
             q2ew = n2ew;
         }
     }
-    if (pd.b >= 0) hist_add<WEIGHTED>(P, hist, pd.b, pd.c, pd.sp, pd.sx, pd.dd, pd.vr);
+    if (pd.b >= 0) hist_add<WEIGHTED, VAR>(P, hist, pd.b, pd.c, pd.sp, pd.sx, pd.dd, pd.vr);
 }
 
-template <int MODE, bool WEIGHTED>
+template <int MODE, bool WEIGHTED, bool VAR>
 __global__ __launch_bounds__(kThreads) void k_pairs(PairArgs P) {
     extern __shared__ __align__(16) char lds[];
     const int lane = threadIdx.x & (kWave - 1);
@@ -695,23 +697,38 @@ __global__ __launch_bounds__(kThreads) void k_pairs(PairArgs P) {
         }
         ShapePtr shp = (ShapePtr)(S.srec + 8 * (int64_t)s0);
         if (shifts & 64)
-            run_item<MODE, SH_WRAP, WEIGHTED>(P, S, p0, np, shifts, shp, ns, hist, T, X, lane, nre);
+            run_item<MODE, SH_WRAP, WEIGHTED, VAR>(P, S, p0, np, shifts, shp, ns, hist, T, X, lane, nre);
         else if (shifts & 63)
-            run_item<MODE, SH_CONST, WEIGHTED>(P, S, p0, np, shifts, shp, ns, hist, T, X, lane, nre);
+            run_item<MODE, SH_CONST, WEIGHTED, VAR>(P, S, p0, np, shifts, shp, ns, hist, T, X, lane, nre);
         else
-            run_item<MODE, SH_NONE, WEIGHTED>(P, S, p0, np, shifts, shp, ns, hist, T, X, lane, nre);
+            run_item<MODE, SH_NONE, WEIGHTED, VAR>(P, S, p0, np, shifts, shp, ns, hist, T, X, lane, nre);
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     if (Sp) flush_hist(P, hist, *Sp, cur_rs, cur_rp, lane);
     if (nre) atomicAdd(P.rechecks, (unsigned long long)nre);
 }
 
-template <int MODE, bool WEIGHTED>
+template <int MODE, bool WEIGHTED, bool VAR>
 void launch_pairs(const PairArgs &A, unsigned grid, size_t lds, hipStream_t st) {
-    MIA_HIP(hipFuncSetAttribute((const void *)k_pairs<MODE, WEIGHTED>,
+    MIA_HIP(hipFuncSetAttribute((const void *)k_pairs<MODE, WEIGHTED, VAR>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((k_pairs<MODE, WEIGHTED>), dim3(grid), dim3(kThreads), lds, st, A);
+    hipLaunchKernelGGL((k_pairs<MODE, WEIGHTED, VAR>), dim3(grid), dim3(kThreads), lds, st, A);
     MIA_HIP(hipGetLastError());
+}
+
+template <int MODE>
+void launch_pairs_mode(const PairArgs &A, unsigned grid, size_t lds, hipStream_t st) {
+    if (A.weighted) {
+        if (A.var)
+            launch_pairs<MODE, true, true>(A, grid, lds, st);
+        else
+            launch_pairs<MODE, true, false>(A, grid, lds, st);
+    } else {
+        if (A.var)
+            launch_pairs<MODE, false, true>(A, grid, lds, st);
+        else
+            launch_pairs<MODE, false, false>(A, grid, lds, st);
+    }
 }
 
 inline unsigned nblk(int64_t n, int t) { return (unsigned)((n + t - 1) / t < 1 ? 1 : (n + t - 1) / t); }
@@ -949,8 +966,8 @@ void count_fast(Ctx &ctx, std::vector<FastSample> &samples, const mia_bins &bins
     double dr = std::fmax(1e-4, 20 * err_r), d2 = std::fmax(1e-4, 20 * err_2);
     // pathological binnings (guard band > 5% of a bin): every pair takes the exact path
     bool exact_only = dr > 0.05 || d2 > 0.05;
-    A.hr = exact_only ? -1.0f : (float)(0.5 - dr);
-    A.h2 = exact_only ? -1.0f : (float)(0.5 - d2);
+    A.dr = exact_only ? 1.0f : (float)dr;
+    A.d2 = exact_only ? 1.0f : (float)d2;
     A.samples = sdd;
     // histogram copies (default 1: the pending-slot registers already absorb
     // most collisions, and a single copy leaves LDS for 8 waves per SIMD);
@@ -987,17 +1004,10 @@ void count_fast(Ctx &ctx, std::vector<FastSample> &samples, const mia_bins &bins
     A.n_waves = waves;
     unsigned grid = (unsigned)((waves + kWavesPerBlock - 1) / kWavesPerBlock);
     if (nitems > 0) {
-        if (bins.mode == MIA_RPPI) {
-            if (A.weighted)
-                launch_pairs<MIA_RPPI, true>(A, grid, lds, st);
-            else
-                launch_pairs<MIA_RPPI, false>(A, grid, lds, st);
-        } else {
-            if (A.weighted)
-                launch_pairs<MIA_RMU, true>(A, grid, lds, st);
-            else
-                launch_pairs<MIA_RMU, false>(A, grid, lds, st);
-        }
+        if (bins.mode == MIA_RPPI)
+            launch_pairs_mode<MIA_RPPI>(A, grid, lds, st);
+        else
+            launch_pairs_mode<MIA_RMU>(A, grid, lds, st);
     }
     MIA_HIP(hipEventRecord(ev[2], st));
     MIA_HIP(hipEventRecord(ev[3], st));
