@@ -364,6 +364,27 @@ class MeasureIABox(SimInfo):
         if return_output:
             return covs, stds
 
+    # -- jackknife covariance across datasets (measure_jackknife.py:485-648) -----------
+    def measure_covariance_multiple_datasets(self, corr_type, dataset_names, num_box=27,
+                                             return_output=False):
+        """Covariance of one dataset's or two datasets' jackknife realisations.
+
+        The reference defines this on its lightcone class; it only reads
+        realisations from the output file, so it serves box outputs as well.
+        """
+        from . import jackknife
+        return jackknife.measure_covariance_multiple_datasets(
+            self.output_file_name, self.snap_group, corr_type, dataset_names, num_box, self.num_bins_r,
+            return_output)
+
+    def create_full_cov_matrix_projections(self, corr_type, dataset_names=["LOS_x", "LOS_y", "LOS_z"],
+                                           num_box=27, return_output=False):
+        """Combined covariance of three projections and of every pair of them."""
+        from . import jackknife
+        return jackknife.create_full_cov_matrix_projections(
+            self.output_file_name, self.snap_group, corr_type, dataset_names, num_box, self.num_bins_r,
+            return_output)
+
     # -- public API ----------------------------------------------------------------
     def _layout(self, temp_file_path):
         if temp_file_path is False:
