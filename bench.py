@@ -44,6 +44,8 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 vector (= FP32 without packed instructions)
 PEAK_FP32_PACKED_TFLOPS = 157.3
+PEAK_HBM_GBPS = 8000.0       # MI355X HBM3E
+SORT_BYTES = (72, 112)       # algorithmic bytes per position / per shape of the counting sort
 FLOP_PER_PAIR = {"rppi": 28, "rmu": 32}   # algorithmic FLOP per in-domain pair
 
 W_BINS = dict(mode="rppi", r=(0.1, 20.0), nr=10, n2=20, pi_max=60.0, jk=3)
@@ -268,6 +270,13 @@ def main():
             f"({time.perf_counter() - tc:.1f} s)")
 
     st = res[0].stats
+    # the counting sort (region ids, cell keys, scan, scatter of every sample) against HBM
+    sort_ms = float(np.mean([s["ms_prepare"] for s in kstats]))
+    sort_bytes = sum(SORT_BYTES[0] * p.n_pos + SORT_BYTES[1] * p.n_shape for p in preps)
+    sort_gbps = sort_bytes / (sort_ms / 1e3) / 1e9
+    cell_sort = {"bound": "hbm", "achieved": sort_gbps, "peak": PEAK_HBM_GBPS, "unit": "GB/s",
+                 "frac": sort_gbps / PEAK_HBM_GBPS, "ms": sort_ms, "bytes": sort_bytes,
+                 "model": "72 B per position + 112 B per shape (read, key, scatter)"}
     line = {
         "metric": "pairs/s & s per wgg+wg+ (1M gal, 27 JK): batched-FP32 vs plain GPU",
         "value": value, "unit": "pairs/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
@@ -283,6 +292,7 @@ def main():
                      "candidate_efficiency": pairs / max(1, st["candidates"]),
                      "model": f"{FLOP_PER_PAIR[w['mode']]} FLOP per in-domain pair; FP64 vector peak "
                               f"(the kernel is vector-ALU + LDS-atomic bound, no matrix work)"},
+        "cell_sort": cell_sort,
         "cpu_baseline": cpu,
         "plain_gpu": plain,
         "one_call_per_sample": loop,

@@ -156,14 +156,25 @@ __global__ __launch_bounds__(kClassifyThreads) void k_classify(
     }
 }
 
-// out[j] += sum over rows r of rows[r*m + j], rows added in order.
-__global__ void k_reduce_rows(const double *__restrict__ rows, int nrows, int m,
-                              double *__restrict__ out) {
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= m) return;
+constexpr int kReduceThreads = 256;
+
+// out[j] += sum over rows r of rows[r*m + j]: one workgroup per column j,
+// thread t adds rows t, t+256, ... in order, then a fixed-shape tree in LDS
+// (the same partial sums in the same order on every run).
+__global__ __launch_bounds__(kReduceThreads) void k_reduce_rows(const double *__restrict__ rows,
+                                                                int nrows, int m,
+                                                                double *__restrict__ out) {
+    __shared__ double part[kReduceThreads];
+    const int j = blockIdx.x;
     double s = 0.0;
-    for (int r = 0; r < nrows; ++r) s += rows[(int64_t)r * m + j];
-    out[j] += s;
+    for (int r = threadIdx.x; r < nrows; r += kReduceThreads) s += rows[(int64_t)r * m + j];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = kReduceThreads / 2; h > 0; h >>= 1) {
+        if (threadIdx.x < h) part[threadIdx.x] += part[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[j] += part[0];
 }
 
 // Counting-sort scatter.  Positions: a, b, l, w (+ reg, key for the plain
@@ -310,7 +321,7 @@ static void sort_one(Ctx &ctx, const double *xyz, int64_t n, const double *w, co
                            e, key0, reg0, counts, rstats, part, which, oor);
         MIA_HIP(hipGetLastError());
         if (local) {
-            hipLaunchKernelGGL(k_reduce_rows, dim3(grid_for(K4, 128)), dim3(128), 0, st, part,
+            hipLaunchKernelGGL(k_reduce_rows, dim3(K4), dim3(kReduceThreads), 0, st, part,
                                (int)grid, K4, rstats);
             MIA_HIP(hipGetLastError());
         }
