@@ -56,6 +56,10 @@ def _chunk(bounds):
     SX = np.zeros((nr, n2))
     DD_jk = np.zeros((max(num_box, 1), nr, n2))
     SP_jk = np.zeros((max(num_box, 1), nr, n2))
+    absum = g["abs_sums"]
+    SPA = np.zeros((nr, n2)) if absum else None
+    SXA = np.zeros((nr, n2)) if absum else None
+    SPA_jk = np.zeros((max(num_box, 1), nr, n2)) if absum else None
     npairs = 0
     shp_i = shp[i:i2]
     tree_pts = shp_i[:, nl] if mode == "rppi" else shp_i
@@ -102,6 +106,9 @@ def _chunk(bounds):
         np.add.at(SX, (ind_r, ind_2), (wp * e_cross[mask]) / (2 * R))
         np.add.at(DD, (ind_r, ind_2), wp)
         npairs += int(mask.sum())
+        if absum:  # tolerance scales for the tests, not part of the timed baseline
+            np.add.at(SPA, (ind_r, ind_2), np.abs((wp * e_plus[mask]) / (2 * R)))
+            np.add.at(SXA, (ind_r, ind_2), np.abs((wp * e_cross[mask]) / (2 * R)))
         if num_box > 0:
             reg_p = jk_pos[idx][mask]
             pm = np.where(reg_p != jk_shape[s])[0]
@@ -109,20 +116,25 @@ def _chunk(bounds):
             np.add.at(SP_jk, (reg_p[pm], ind_r[pm], ind_2[pm]), wp[pm] * e_plus[mask][pm])
             np.add.at(DD_jk, (jk_shape[s], ind_r, ind_2), wp)
             np.add.at(DD_jk, (reg_p[pm], ind_r[pm], ind_2[pm]), wp[pm])
-    return SP, SX, DD, DD_jk, SP_jk, npairs
+            if absum:
+                np.add.at(SPA_jk, (jk_shape[s], ind_r, ind_2), np.abs(wp * e_plus[mask]))
+                np.add.at(SPA_jk, (reg_p[pm], ind_r[pm], ind_2[pm]), np.abs(wp[pm] * e_plus[mask][pm]))
+    return SP, SX, DD, DD_jk, SP_jk, npairs, SPA, SXA, SPA_jk
 
 
 def count_pairs(prep, r_bins, second_bins, r_min, r_max, boxsize, *, mode="rppi",
                 jk_pos=None, jk_shape=None, num_box=0, periodic=True, rp_cut=None,
-                chunk_size=1000, num_nodes=1, chunk_subset=None):
+                chunk_size=1000, num_nodes=1, chunk_subset=None, abs_sums=False):
     """Tree counter over all shape chunks, or over ``chunk_subset`` (chunk numbers).
 
     Returns the same keys as oracle.brute.count_pairs (without 'variance',
-    which the reference's tree path never accumulates), plus 'seconds' of
-    wall time for the counting itself and 'chunks' = (done, total).
+    which the reference's tree path never accumulates, and without the
+    per-bin sums of |terms| unless ``abs_sums``), plus 'seconds' of wall time
+    for the counting itself and 'chunks' = (done, total).
     """
     _setup(prep, r_bins, second_bins, r_min, r_max, boxsize, mode, jk_pos, jk_shape, num_box,
            periodic, rp_cut)
+    _G["abs_sums"] = abs_sums
     Ns = len(prep["shapes"])
     starts = np.arange(0, Ns, chunk_size)
     bounds = [(int(s), int(min(Ns, s + chunk_size))) for s in starts]
@@ -144,6 +156,7 @@ def count_pairs(prep, r_bins, second_bins, r_min, r_max, boxsize, *, mode="rppi"
     DD_jk = np.zeros((max(num_box, 1), nr, n2))
     SP_jk = np.zeros((max(num_box, 1), nr, n2))
     npairs = 0
+    SPA, SXA, SPA_jk = np.zeros((nr, n2)), np.zeros((nr, n2)), np.zeros((max(num_box, 1), nr, n2))
     for r in result:
         SP += r[0]
         SX += r[1]
@@ -151,10 +164,18 @@ def count_pairs(prep, r_bins, second_bins, r_min, r_max, boxsize, *, mode="rppi"
         DD_jk += r[3]
         SP_jk += r[4]
         npairs += r[5]
+        if abs_sums:
+            SPA += r[6]
+            SXA += r[7]
+            SPA_jk += r[8]
     out = dict(DD=DD, SplusD=SP, ScrossD=SX, npairs=npairs, seconds=secs, chunks=(len(bounds), total))
+    if abs_sums:
+        out["SplusD_abs"], out["ScrossD_abs"] = SPA, SXA
     if num_box > 0:
         out["DD_jk"] = DD_jk
         out["SplusD_jk"] = SP_jk
+        if abs_sums:
+            out["SplusD_jk_abs"] = SPA_jk
     return out
 
 
