@@ -144,6 +144,10 @@ class H5Store:
         self.filename = str(filename)
         self.mode = mode
         self._h5py = None
+        # groups known to exist, and the datasets this store wrote (served
+        # back by read() without a file round trip; the file holds the same)
+        self._groups = {""}
+        self._written = {}
         lib = _load()
         if lib is None:
             try:
@@ -258,10 +262,11 @@ class H5Store:
     def require_group(self, path: str) -> str:
         """mkdir -p of a group path (write_data.py:23-56); returns the path."""
         path = _norm(path)
-        if path == "":
+        if path in self._groups:
             return path
         if self._h5py is not None:
             self._h5py.require_group(path)
+            self._groups.add(path)
             return path
         parts = path.split("/")
         for i in range(1, len(parts) + 1):
@@ -273,6 +278,8 @@ class H5Store:
             if gid < 0:
                 raise OSError(f"cannot create group {sub!r}")
             self._lib.H5Gclose(gid)
+        for i in range(1, len(parts) + 1):
+            self._groups.add("/".join(parts[:i]))
         return path
 
     # -- datasets ------------------------------------------------------------
@@ -286,6 +293,7 @@ class H5Store:
             arr = np.ascontiguousarray(arr, dtype=np.int64)
         else:
             arr = np.ascontiguousarray(arr, dtype=np.float64)
+        self._written[path] = arr.copy()
         if self._h5py is not None:
             if path in self._h5py:
                 del self._h5py[path]
@@ -317,6 +325,8 @@ class H5Store:
 
     def read(self, path: str) -> np.ndarray:
         path = _norm(path)
+        if path in self._written:
+            return self._written[path].copy()
         if self._h5py is not None:
             return np.asarray(self._h5py[path][()])
         lib = self._lib
