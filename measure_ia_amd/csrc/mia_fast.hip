@@ -80,9 +80,9 @@ struct PlanArgs {
     const Block *blocks;
     int nblocks;
     int blk_base;        // global index of this sample's first block
-    int32_t *key_count;  // count pass
-    int32_t *key_cursor; // emit pass
-    const int32_t *key_off;
+    unsigned long long *key_count;   // count pass
+    unsigned long long *key_cursor;  // emit pass
+    const int64_t *key_off;
     Item *items;
     int64_t *cost;       // emit pass: balancing cost of every item
     unsigned long long *candidates;  // emit pass: sum of ns * np
@@ -229,15 +229,21 @@ constexpr int64_t kItemOverhead = 256;  // per-item cost, in pair evaluations
 // atomic per workgroup instead of one per range; the emit pass enumerates the
 // ranges twice (count, then place) around that reservation.
 __global__ __launch_bounds__(kPlanThreads) void k_plan_items(PlanArgs P) {
-    extern __shared__ int32_t lkey[];  // local: counts [nk], reserved bases [nk]
+    // local: reserved bases [nk] (64-bit), then counts [nk]
+    extern __shared__ int64_t lmem[];
     const int nk = P.K1 * P.K1;
+    int64_t *lbase = lmem;
+    int32_t *lcnt = (int32_t *)(lmem + nk);
     const int kb = P.sample * nk;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = t < (int64_t)P.nblocks * P.nxw * P.nxw;
     const bool local = P.local != 0;
     const int pmax = P.pmax;
     if (local) {
-        for (int j = threadIdx.x; j < 2 * nk; j += blockDim.x) lkey[j] = 0;
+        for (int j = threadIdx.x; j < nk; j += blockDim.x) {
+            lcnt[j] = 0;
+            lbase[j] = 0;
+        }
         __syncthreads();
     }
     if (!P.emit || local) {
@@ -245,20 +251,20 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan_items(PlanArgs P) {
             plan_ranges(P, t, [&](int k, int pb, int pe, int, uint8_t, int, int) {
                 int n = (pe - pb + pmax - 1) / pmax;
                 if (local)
-                    atomicAdd(&lkey[k], n);
+                    atomicAdd(&lcnt[k], n);
                 else
-                    atomicAdd(&P.key_count[kb + k], n);
+                    atomicAdd(&P.key_count[kb + k], (unsigned long long)n);
             });
     }
     if (local) {
         __syncthreads();
         for (int j = threadIdx.x; j < nk; j += blockDim.x) {
-            int c = lkey[j];
+            int c = lcnt[j];
             if (!P.emit) {
-                if (c) atomicAdd(&P.key_count[kb + j], c);
+                if (c) atomicAdd(&P.key_count[kb + j], (unsigned long long)c);
             } else {
-                lkey[nk + j] = c ? atomicAdd(&P.key_cursor[kb + j], c) : 0;
-                lkey[j] = 0;
+                lbase[j] = c ? (int64_t)atomicAdd(&P.key_cursor[kb + j], (unsigned long long)c) : 0;
+                lcnt[j] = 0;
             }
         }
         __syncthreads();
@@ -270,8 +276,9 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan_items(PlanArgs P) {
     unsigned long long cand = 0;
     if (active) plan_ranges(P, t, [&](int k, int pb, int pe, int rp, uint8_t sh, int blk, int ns) {
         int n = (pe - pb + pmax - 1) / pmax;
-        int slot = P.key_off[kb + k] +
-                   (local ? lkey[nk + k] + atomicAdd(&lkey[k], n) : atomicAdd(&P.key_cursor[kb + k], n));
+        int64_t slot = P.key_off[kb + k] +
+                       (local ? lbase[k] + (int64_t)atomicAdd(&lcnt[k], n)
+                              : (int64_t)atomicAdd(&P.key_cursor[kb + k], (unsigned long long)n));
         for (int q = 0; q < n; ++q) {
             int p = pb + q * pmax;
             Item it;
@@ -813,13 +820,13 @@ void count_fast(Ctx &ctx, std::vector<FastSample> &samples, const mia_bins &bins
     // ---- plan: work items, counting-sorted by (sample, R_s, R_p)
     const int K1 = K1max;
     const int64_t nkeys = (int64_t)ns_ * K1 * K1;
-    const size_t plan_lds = (size_t)2 * K1 * K1 * sizeof(int32_t);
-    const int plan_local = plan_lds <= 48 * 1024 ? 1 : 0;
-    int32_t *kc = ctx.arena.get<int32_t>("fast/keycount", nkeys + 1);
-    int32_t *ko = ctx.arena.get<int32_t>("fast/keyoff", nkeys + 1);
-    int32_t *kcur = ctx.arena.get<int32_t>("fast/keycur", nkeys + 1);
-    fill<int32_t>(kc, 0, nkeys + 1, st);
-    fill<int32_t>(kcur, 0, nkeys + 1, st);
+    const size_t plan_lds = (size_t)K1 * K1 * (sizeof(int64_t) + sizeof(int32_t));
+    const int plan_local = plan_lds <= 64 * 1024 ? 1 : 0;
+    unsigned long long *kc = ctx.arena.get<unsigned long long>("fast/keycount", nkeys + 1);
+    int64_t *ko = ctx.arena.get<int64_t>("fast/keyoff", nkeys + 1);
+    unsigned long long *kcur = ctx.arena.get<unsigned long long>("fast/keycur", nkeys + 1);
+    fill<unsigned long long>(kc, 0ull, nkeys + 1, st);
+    fill<unsigned long long>(kcur, 0ull, nkeys + 1, st);
     std::vector<PlanArgs> plans(ns_);
     for (int si = 0; si < ns_; ++si) {
         FastSample &F = samples[si];
@@ -856,11 +863,10 @@ void count_fast(Ctx &ctx, std::vector<FastSample> &samples, const mia_bins &bins
             MIA_HIP(hipGetLastError());
         }
     }
-    exclusive_scan<int32_t>(ctx, kc, ko, nkeys, st, "fast/keys", 0);
-    int32_t nitems32 = 0;
-    MIA_HIP(hipMemcpyAsync(&nitems32, ko + nkeys, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    exclusive_scan<int64_t>(ctx, (const int64_t *)kc, ko, nkeys, st, "fast/keys", 0);
+    int64_t nitems = 0;
+    MIA_HIP(hipMemcpyAsync(&nitems, ko + nkeys, sizeof(int64_t), hipMemcpyDeviceToHost, st));
     MIA_HIP(hipStreamSynchronize(st));
-    const int64_t nitems = nitems32;
     Item *items = ctx.arena.get<Item>("fast/items", (size_t)std::max<int64_t>(nitems, 1));
     int64_t *cost = ctx.arena.get<int64_t>("fast/cost", nitems + 1);
     int64_t *cpre = ctx.arena.get<int64_t>("fast/costpre", nitems + 1);
