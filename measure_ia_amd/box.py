@@ -109,6 +109,30 @@ class MeasureIABox(SimInfo):
 
     # -- reference static helpers (measure_IA_base.py:229-351) -----------------
     @staticmethod
+    def calculate_dot_product_arrays(a1, a2):
+        """Row-wise dot product, columns added in order (measure_IA_base.py:186-206)."""
+        dot_product = np.zeros(np.shape(a1)[0])
+        for i in range(np.shape(a1)[1]):
+            dot_product += a1[:, i] * a2[:, i]
+        return dot_product
+
+    @staticmethod
+    def get_ellipticity(e, phi):
+        """e_+ = e cos 2phi, e_x = e sin 2phi (measure_IA_base.py:208-227)."""
+        return e * np.cos(2 * phi), e * np.sin(2 * phi)
+
+    @staticmethod
+    def setdiff2D(a1, a2):
+        """Row-wise np.setdiff1d of two nested lists (measure_IA_base.py:353-375)."""
+        assert len(a1) == len(a2), "Lengths of lists where each row is to be compared, are not the same."
+        return [np.setdiff1d(a1[i], a2[i]) for i in range(len(a1))]
+
+    @staticmethod
+    def setdiff_omit(a1, a2, incl_ind):
+        """np.setdiff1d(a1[i], a2) for the rows i in incl_ind (measure_IA_base.py:377-401)."""
+        return [np.setdiff1d(a1[i], a2) for i in range(len(a1)) if np.isin(i, incl_ind)]
+
+    @staticmethod
     def get_random_pairs(rp_max, rp_min, pi_max, pi_min, L3, corrtype, Num_position, Num_shape):
         if corrtype == "auto":
             return (Num_position - 1.0) * Num_shape / 2.0 * np.pi * (rp_max ** 2 - rp_min ** 2) * abs(

@@ -54,21 +54,17 @@ class ReadData:
             def get(path):
                 return f.read(base + path)
 
+            # assignments happen one by one inside each try, as in the reference:
+            # a missing covariance leaves the values already read in place
             for grp, attr, sep, sep_attr in (("multipoles_gg", "multipoles_gg", "_r", "r"),
                                              ("multipoles_g_plus", "multipoles_gp", "_r", "r"),
                                              ("w_gg", "w_gg", "_rp", "rp"),
                                              ("w_g_plus", "w_gp", "_rp", "rp")):
                 try:
-                    val = get(f"{grp}/{dataset_name}")
-                    sv = get(f"{grp}/{dataset_name}{sep}")
-                    cov = err = None
+                    setattr(self, attr, get(f"{grp}/{dataset_name}"))
+                    setattr(self, sep_attr, get(f"{grp}/{dataset_name}{sep}"))
                     if num_jk is not None:
-                        cov = get(f"{grp}/{dataset_name}_jackknife_cov_{num_jk}")
-                        err = get(f"{grp}/{dataset_name}_jackknife_{num_jk}")
+                        setattr(self, f"cov_{attr}", get(f"{grp}/{dataset_name}_jackknife_cov_{num_jk}"))
+                        setattr(self, f"errors_{attr}", get(f"{grp}/{dataset_name}_jackknife_{num_jk}"))
                 except KeyError:
-                    continue
-                setattr(self, attr, val)
-                setattr(self, sep_attr, sv)
-                if num_jk is not None:
-                    setattr(self, f"cov_{attr}", cov)
-                    setattr(self, f"errors_{attr}", err)
+                    pass

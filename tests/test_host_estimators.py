@@ -103,3 +103,36 @@ def test_estimates_match_oracle_estimator():
         np.testing.assert_allclose(est.real_SplusD[i], ref[("jk", i)]["SplusD"], rtol=1e-13)
         np.testing.assert_array_equal(est.real_xi_gg[i], ref[("jk", i)]["xi_gg"])
         np.testing.assert_array_equal(est.real_DD[i], ref[("jk", i)]["DD"])
+
+
+def test_box_static_helpers():
+    """measure_IA_base.py:186-227, 353-401: the helpers a MeasureIABox exposes."""
+    from measure_ia_amd import MeasureIABox
+    rng = np.random.default_rng(4)
+    a, b = rng.normal(size=(50, 3)), rng.normal(size=(50, 3))
+    dot = MeasureIABox.calculate_dot_product_arrays(a, b)
+    np.testing.assert_array_equal(dot, (a[:, 0] * b[:, 0] + a[:, 1] * b[:, 1]) + a[:, 2] * b[:, 2])
+    ep, ex = MeasureIABox.get_ellipticity(np.array([0.5, 0.5, 0.5]), np.array([0.0, np.pi / 4, np.pi / 2]))
+    np.testing.assert_allclose(ep, [0.5, 0.0, -0.5], atol=1e-16)
+    np.testing.assert_allclose(ex, [0.0, 0.5, 0.0], atol=1e-16)
+    d = MeasureIABox.setdiff2D([[1, 2, 3], [4, 5]], [[2], [5, 6]])
+    assert [list(x) for x in d] == [[1, 3], [4]]
+    d = MeasureIABox.setdiff_omit([[1, 2, 3], [4, 5], [6, 7]], [2, 6], [0, 2])
+    assert [list(x) for x in d] == [[1, 3], [7]]
+    with pytest.raises(AssertionError):
+        MeasureIABox.setdiff2D([[1]], [[1], [2]])
+
+
+def test_read_output_keeps_values_when_covariance_missing(tmp_path):
+    """read_data.py:336-374: assignments inside each try happen one by one."""
+    from measure_ia_amd import ReadData
+    from measure_ia_amd.hdf5io import H5Store
+    fn = tmp_path / "cat.hdf5"
+    with H5Store(str(fn), "a") as st:
+        st.write("Snapshot_99/w_gg", "All", np.arange(4.0))
+        st.write("Snapshot_99/w_gg", "All_rp", np.arange(4.0) + 1)
+    rd = ReadData("TNG300", "cat", 99, data_path=str(tmp_path))
+    rd.read_MeasureIA_output("All", 27)
+    np.testing.assert_array_equal(rd.w_gg, np.arange(4.0))
+    np.testing.assert_array_equal(rd.rp, np.arange(4.0) + 1)
+    assert rd.cov_w_gg is None and rd.w_gp is None and rd.multipoles_gg is None
