@@ -125,19 +125,14 @@ int mia_count(mia_ctx *ctx, const mia_sample *sample, const mia_bins *bins, cons
         mia::Grid g = mia::make_grid(*sample, *bins, tuning);
         mia::Sorted pos, shp;
         int *oor = ctx->c.arena.get<int>("oor", 1);
-        hipEvent_t e0, e1;
-        MIA_HIP(hipEventCreate(&e0));
-        MIA_HIP(hipEventCreate(&e1));
-        MIA_HIP(hipEventRecord(e0, st));
+        mia::Events ev(2);
+        MIA_HIP(hipEventRecord(ev[0], st));
         mia::fill<int32_t>(oor, 0, 1, st);
         mia::prepare_sample(ctx->c, *sample, g, pos, shp, out->region_stats, oor,
                             method == MIA_PLAIN, st, "s0");
-        MIA_HIP(hipEventRecord(e1, st));
-        mia::check_prepared(oor, 1, st);
-        float ms = 0;
-        MIA_HIP(hipEventElapsedTime(&ms, e0, e1));
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
+        MIA_HIP(hipEventRecord(ev[1], st));
+        // the fast path reads the range flag with its first readback
+        if (method == MIA_PLAIN) mia::check_prepared(oor, 1, st);
         int K = g.jk > 0 ? g.jk * g.jk * g.jk : 0;
         int64_t nb = (int64_t)(1 + K) * bins->n_r * bins->n_2;
         if (method == MIA_PLAIN) {
@@ -150,10 +145,11 @@ int mia_count(mia_ctx *ctx, const mia_sample *sample, const mia_bins *bins, cons
             v[0].shp = shp;
             v[0].out = out;
             v[0].weighted = weighted;
-            mia::count_fast(ctx->c, v, *bins, tuning, st, stats);
+            mia::count_fast(ctx->c, v, *bins, tuning, st, stats, oor);
         }
         unit_dd(*out, weighted, nb, st);
         MIA_HIP(hipStreamSynchronize(st));
+        float ms = ev.ms(0, 1);
         if (stats) {
             stats->ms_prepare = ms;
             stats->columns_per_side = g.nc;
@@ -173,10 +169,8 @@ int mia_count_batch(mia_ctx *ctx, int32_t n, const mia_sample *samples, const mi
         if (stats) std::memset(stats, 0, sizeof(*stats));
         std::vector<mia::FastSample> v(n);
         int *oor = ctx->c.arena.get<int>("oor", n);
-        hipEvent_t e0, e1;
-        MIA_HIP(hipEventCreate(&e0));
-        MIA_HIP(hipEventCreate(&e1));
-        MIA_HIP(hipEventRecord(e0, st));
+        mia::Events ev(2);
+        MIA_HIP(hipEventRecord(ev[0], st));
         mia::fill<int32_t>(oor, 0, n, st);
         for (int i = 0; i < n; ++i) {
             check_sample(&samples[i]);
@@ -189,18 +183,14 @@ int mia_count_batch(mia_ctx *ctx, int32_t n, const mia_sample *samples, const mi
             mia::prepare_sample(ctx->c, samples[i], v[i].g, v[i].pos, v[i].shp,
                                 outs[i].region_stats, oor + i, false, st, "s" + std::to_string(i));
         }
-        MIA_HIP(hipEventRecord(e1, st));
-        mia::check_prepared(oor, n, st);
-        float ms = 0;
-        MIA_HIP(hipEventElapsedTime(&ms, e0, e1));
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        mia::count_fast(ctx->c, v, *bins, tuning, st, stats);
+        MIA_HIP(hipEventRecord(ev[1], st));
+        mia::count_fast(ctx->c, v, *bins, tuning, st, stats, oor);
         for (int i = 0; i < n; ++i) {
             int K = v[i].g.jk > 0 ? v[i].g.jk * v[i].g.jk * v[i].g.jk : 0;
             unit_dd(outs[i], v[i].weighted, (int64_t)(1 + K) * bins->n_r * bins->n_2, st);
         }
         MIA_HIP(hipStreamSynchronize(st));
+        float ms = ev.ms(0, 1);
         if (stats) {
             stats->ms_prepare = ms;
             stats->columns_per_side = v[0].g.nc;

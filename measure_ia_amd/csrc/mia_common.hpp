@@ -102,6 +102,30 @@ struct Sorted {
 
 struct Ctx;
 
+// HIP events destroyed on scope exit (also when a count throws).
+class Events {
+  public:
+    explicit Events(int n) : ev_(n, nullptr) {
+        for (auto &e : ev_) MIA_HIP(hipEventCreate(&e));
+    }
+    ~Events() {
+        for (auto e : ev_)
+            if (e) (void)hipEventDestroy(e);
+    }
+    Events(const Events &) = delete;
+    Events &operator=(const Events &) = delete;
+    hipEvent_t operator[](int i) const { return ev_[i]; }
+    // milliseconds between two recorded events (the later one must be complete)
+    float ms(int a, int b) const {
+        float t = 0;
+        MIA_HIP(hipEventElapsedTime(&t, ev_[a], ev_[b]));
+        return t;
+    }
+
+  private:
+    std::vector<hipEvent_t> ev_;
+};
+
 // Host-side helpers implemented in the .hip files.
 void region_ids(Ctx &ctx, const double *xyz, int64_t n, double jk_sub, int jk, int32_t *region,
                 hipStream_t st);
@@ -124,8 +148,9 @@ struct FastSample {
     const mia_counts *out;
     bool weighted;
 };
+// oor: device out-of-box flags of the samples (checked with the first readback), or NULL
 void count_fast(Ctx &ctx, std::vector<FastSample> &samples, const mia_bins &bins,
-                const mia_tuning *tuning, hipStream_t st, mia_stats *stats);
+                const mia_tuning *tuning, hipStream_t st, mia_stats *stats, const int *oor);
 
 struct Ctx {
     int device = 0;

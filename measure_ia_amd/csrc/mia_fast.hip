@@ -350,7 +350,7 @@ struct PairArgs {
     const Block *blocks;
     const int64_t *cost_prefix;  // exclusive, n_items + 1
     int64_t n_items;
-    int64_t total_cost;
+    const int64_t *total_cost;   // device: cost prefix total (read by the kernel)
     int n_waves;
     int mode, n_r, n_2, weighted, var, nbins;
     const double *T, *X;         // exact thresholds (global copy)
@@ -661,6 +661,7 @@ __global__ __launch_bounds__(kThreads) void k_pairs(PairArgs P) {
     const int w = blockIdx.x * kWavesPerBlock + wv;
     if (w >= P.n_waves || P.n_items == 0) return;
     const MIA_CONST int64_t *cpre = (const MIA_CONST int64_t *)P.cost_prefix;
+    const double total_cost = (double)*(const MIA_CONST int64_t *)P.total_cost;
     // items [it0, it1) whose cost prefix falls in this wave's share
     auto first_at_least = [&](int64_t target) {
         int64_t lo = 0, hi = P.n_items;
@@ -668,7 +669,7 @@ __global__ __launch_bounds__(kThreads) void k_pairs(PairArgs P) {
             int64_t mid = (lo + hi) >> 1;
             // prefix[mid] * W >= target * C (any monotone rule works: the
             // neighbouring wave evaluates the same boundary)
-            if ((double)cpre[mid] * P.n_waves >= (double)target * (double)P.total_cost)
+            if ((double)cpre[mid] * P.n_waves >= (double)target * total_cost)
                 hi = mid;
             else
                 lo = mid + 1;
@@ -743,7 +744,7 @@ inline unsigned nblk(int64_t n, int t) { return (unsigned)((n + t - 1) / t < 1 ?
 }  // namespace
 
 void count_fast(Ctx &ctx, std::vector<FastSample> &samples, const mia_bins &bins,
-                const mia_tuning *tuning, hipStream_t st, mia_stats *stats) {
+                const mia_tuning *tuning, hipStream_t st, mia_stats *stats, const int *oor) {
     const int ns_ = (int)samples.size();
     const int n_r = bins.n_r, n_2 = bins.n_2, B = n_r * n_2;
     MIA_REQUIRE(n_r >= 1 && n_r <= kMaxBinsR && n_2 >= 1 && n_2 <= kMaxBins2, "bin counts out of range");
@@ -759,15 +760,14 @@ void count_fast(Ctx &ctx, std::vector<FastSample> &samples, const mia_bins &bins
     MIA_HIP(hipMemcpyAsync(T, bins.r2_edges, sizeof(double) * (n_r + 1), hipMemcpyHostToDevice, st));
     MIA_HIP(hipMemcpyAsync(X, xh.data(), sizeof(double) * (n_2 + 1), hipMemcpyHostToDevice, st));
 
-    hipEvent_t ev[4];
-    for (auto &e : ev) MIA_HIP(hipEventCreate(&e));
+    Events ev(4);
     MIA_HIP(hipEventRecord(ev[0], st));
 
     // ---- plan: shape blocks per sample (one host sync for all samples)
     const int bsz = tuning && tuning->block_shapes > 0 ? std::min(tuning->block_shapes, kMaxBlockShapes) : kMaxBlockShapes;
     const int pmax = tuning && tuning->item_positions > 0 ? tuning->item_positions : 1024;
     std::vector<int> blk_base(ns_ + 1, 0), gsz(ns_), ngr(ns_);
-    int32_t *btot = ctx.arena.get<int32_t>("fast/btot", ns_);
+    int32_t *btot = ctx.arena.get<int32_t>("fast/btot", 2 * (size_t)ns_);
     int K1max = 1;
     for (int si = 0; si < ns_; ++si) {
         FastSample &F = samples[si];
@@ -800,9 +800,17 @@ void count_fast(Ctx &ctx, std::vector<FastSample> &samples, const mia_bins &bins
         MIA_HIP(hipMemcpyAsync(btot + si, nbo + nt, sizeof(int32_t), hipMemcpyDeviceToDevice, st));
     }
     {
-        std::vector<int32_t> h(ns_);
-        MIA_HIP(hipMemcpyAsync(h.data(), btot, sizeof(int32_t) * ns_, hipMemcpyDeviceToHost, st));
+        // one readback: block totals and the samples' out-of-box flags
+        std::vector<int32_t> h(2 * ns_, 0);
+        if (oor)
+            MIA_HIP(hipMemcpyAsync(btot + ns_, oor, sizeof(int32_t) * ns_, hipMemcpyDeviceToDevice, st));
+        MIA_HIP(hipMemcpyAsync(h.data(), btot, sizeof(int32_t) * (oor ? 2 * ns_ : ns_),
+                               hipMemcpyDeviceToHost, st));
         MIA_HIP(hipStreamSynchronize(st));
+        for (int si = 0; si < ns_; ++si)
+            if (h[ns_ + si])
+                throw Error{MIA_E_RANGE, "coordinates outside [0, boxsize] (or NaN) in sample " +
+                                             std::to_string(si)};
         for (int si = 0; si < ns_; ++si) blk_base[si + 1] = blk_base[si] + h[si];
     }
     const int nblocks = blk_base[ns_];
@@ -870,8 +878,10 @@ void count_fast(Ctx &ctx, std::vector<FastSample> &samples, const mia_bins &bins
     Item *items = ctx.arena.get<Item>("fast/items", (size_t)std::max<int64_t>(nitems, 1));
     int64_t *cost = ctx.arena.get<int64_t>("fast/cost", nitems + 1);
     int64_t *cpre = ctx.arena.get<int64_t>("fast/costpre", nitems + 1);
-    unsigned long long *ncand = ctx.arena.get<unsigned long long>("fast/candidates", 1);
-    fill<unsigned long long>(ncand, 0ull, 1, st);
+    // [0] rechecks, [1] candidates: read back together after the pair kernel
+    unsigned long long *counters = ctx.arena.get<unsigned long long>("fast/counters", 2);
+    fill<unsigned long long>(counters, 0ull, 2, st);
+    unsigned long long *ncand = counters + 1;
     for (int si = 0; si < ns_; ++si) {
         PlanArgs P = plans[si];
         P.emit = 1;
@@ -886,9 +896,6 @@ void count_fast(Ctx &ctx, std::vector<FastSample> &samples, const mia_bins &bins
         }
     }
     exclusive_scan<int64_t>(ctx, cost, cpre, nitems, st, "fast/cost", 0);
-    int64_t total_cost = 0;
-    MIA_HIP(hipMemcpyAsync(&total_cost, cpre + nitems, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-    MIA_HIP(hipStreamSynchronize(st));
     MIA_HIP(hipEventRecord(ev[1], st));
 
     // ---- accumulators
@@ -919,12 +926,11 @@ void count_fast(Ctx &ctx, std::vector<FastSample> &samples, const mia_bins &bins
         fill<double>(S.sp, 0.0, nb, st);
         fill<double>(S.sx, 0.0, B, st);
         if (S.dd) fill<double>(S.dd, 0.0, nb, st);
-        if (S.var) fill<double>(S.var, 0.0, B, st);
+        if (F.out->variance) fill<double>(F.out->variance, 0.0, B, st);
     }
     SampleDev *sdd = ctx.arena.get<SampleDev>("fast/samples", ns_);
     MIA_HIP(hipMemcpyAsync(sdd, sd.data(), sizeof(SampleDev) * ns_, hipMemcpyHostToDevice, st));
-    unsigned long long *rechecks = ctx.arena.get<unsigned long long>("fast/rechecks", 1);
-    fill<unsigned long long>(rechecks, 0ull, 1, st);
+    unsigned long long *rechecks = counters;
 
     // ---- pair kernel
     PairArgs A{};
@@ -932,7 +938,7 @@ void count_fast(Ctx &ctx, std::vector<FastSample> &samples, const mia_bins &bins
     A.blocks = blocks;
     A.cost_prefix = cpre;
     A.n_items = nitems;
-    A.total_cost = total_cost;
+    A.total_cost = cpre + nitems;
     A.mode = bins.mode;
     A.n_r = n_r;
     A.n_2 = n_2;
@@ -1018,14 +1024,10 @@ void count_fast(Ctx &ctx, std::vector<FastSample> &samples, const mia_bins &bins
     MIA_HIP(hipEventRecord(ev[2], st));
     MIA_HIP(hipEventRecord(ev[3], st));
     MIA_HIP(hipEventSynchronize(ev[3]));
-    float t01 = 0, t12 = 0, t23 = 0;
-    MIA_HIP(hipEventElapsedTime(&t01, ev[0], ev[1]));
-    MIA_HIP(hipEventElapsedTime(&t12, ev[1], ev[2]));
-    MIA_HIP(hipEventElapsedTime(&t23, ev[2], ev[3]));
-    unsigned long long nre = 0, ncand_h = 0;
-    MIA_HIP(hipMemcpy(&nre, rechecks, sizeof(nre), hipMemcpyDeviceToHost));
-    MIA_HIP(hipMemcpy(&ncand_h, ncand, sizeof(ncand_h), hipMemcpyDeviceToHost));
-    for (auto &e : ev) (void)hipEventDestroy(e);
+    const float t01 = ev.ms(0, 1), t12 = ev.ms(1, 2), t23 = ev.ms(2, 3);
+    unsigned long long hc[2] = {0, 0};
+    MIA_HIP(hipMemcpy(hc, counters, sizeof(hc), hipMemcpyDeviceToHost));
+    const unsigned long long nre = hc[0], ncand_h = hc[1];
     if (stats) {
         stats->ms_plan = t01;
         stats->ms_pairs = t12;

@@ -226,22 +226,16 @@ void count_plain(Ctx &ctx, const mia_sample &s, const mia_bins &bins, const Grid
     fill<double>(out.splus, 0.0, nb, st);
     fill<double>(out.scross, 0.0, A.nbins, st);
     if (out.variance) fill<double>(out.variance, 0.0, A.nbins, st);
-    hipEvent_t e0, e1;
-    MIA_HIP(hipEventCreate(&e0));
-    MIA_HIP(hipEventCreate(&e1));
-    MIA_HIP(hipEventRecord(e0, st));
+    Events ev(2);
+    MIA_HIP(hipEventRecord(ev[0], st));
     if (shp.n > 0 && pos.n > 0) {
         unsigned blocks = (unsigned)((shp.n + 255) / 256);
         hipLaunchKernelGGL(k_plain, dim3(blocks), dim3(256), 0, st, A, pos, shp);
         MIA_HIP(hipGetLastError());
     }
-    MIA_HIP(hipEventRecord(e1, st));
-    MIA_HIP(hipEventSynchronize(e1));
-    float ms = 0;
-    MIA_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    if (stats) stats->ms_pairs = ms;
+    MIA_HIP(hipEventRecord(ev[1], st));
+    MIA_HIP(hipEventSynchronize(ev[1]));
+    if (stats) stats->ms_pairs = ev.ms(0, 1);
 }
 
 }  // namespace mia

@@ -138,26 +138,41 @@ class Engine:
         return ms, keep, (wp is not None or ws is not None)
 
     def _outputs(self, K, nb, variance):
+        """All outputs in one uninitialised device buffer (the count call zeroes
+        and fills every one of them); returns (layout, buffer, MiaCounts)."""
         t = self.torch
-        o = dict(npairs=t.zeros((1 + K) * nb, dtype=t.int64, device=self.dev),
-                 dd=t.zeros((1 + K) * nb, dtype=t.float64, device=self.dev),
-                 splus=t.zeros((1 + K) * nb, dtype=t.float64, device=self.dev),
-                 scross=t.zeros(nb, dtype=t.float64, device=self.dev),
-                 variance=t.zeros(nb, dtype=t.float64, device=self.dev) if variance else None,
-                 region_stats=t.zeros(max(K, 1) * 4, dtype=t.float64, device=self.dev) if K else None)
-        c = _capi.MiaCounts(**{k: (None if v is None else v.data_ptr()) for k, v in o.items()})
-        return o, c
+        sizes = [("npairs", (1 + K) * nb), ("dd", (1 + K) * nb), ("splus", (1 + K) * nb),
+                 ("scross", nb), ("variance", nb if variance else 0),
+                 ("region_stats", 4 * K)]
+        total = sum(n for _, n in sizes)
+        buf = t.empty(max(total, 1), dtype=t.float64, device=self.dev)
+        layout, off, ptrs = {}, 0, {}
+        for name, n in sizes:
+            layout[name] = (off, n) if n else None
+            ptrs[name] = buf.data_ptr() + 8 * off if n else None
+            off += n
+        return layout, buf, _capi.MiaCounts(**ptrs)
 
     @staticmethod
     def _host(o, K, th: Thresholds, stats):
+        layout, buf = o
+        host = buf.cpu().numpy()  # one device-to-host copy
         shp3 = (1 + K, th.n_r, th.n_2)
+
+        def part(name, shape, dtype=None):
+            if layout[name] is None:
+                return None
+            off, n = layout[name]
+            a = host[off:off + n]
+            return (a.view(dtype) if dtype else a.copy()).reshape(shape).copy()
+
         return CountResult(
-            npairs=o["npairs"].cpu().numpy().reshape(shp3),
-            dd=o["dd"].cpu().numpy().reshape(shp3),
-            splus=o["splus"].cpu().numpy().reshape(shp3),
-            scross=o["scross"].cpu().numpy().reshape(th.n_r, th.n_2),
-            variance=None if o["variance"] is None else o["variance"].cpu().numpy().reshape(th.n_r, th.n_2),
-            region_stats=None if o["region_stats"] is None else o["region_stats"].cpu().numpy().reshape(K, 4),
+            npairs=part("npairs", shp3, np.int64),
+            dd=part("dd", shp3),
+            splus=part("splus", shp3),
+            scross=part("scross", (th.n_r, th.n_2)),
+            variance=part("variance", (th.n_r, th.n_2)),
+            region_stats=part("region_stats", (K, 4)),
             stats=stats)
 
     # -- API -----------------------------------------------------------------
@@ -168,7 +183,7 @@ class Engine:
         K = sample.jk_per_side ** 3 if sample.jk_per_side > 0 else 0
         nb = th.n_r * th.n_2
         bins, keep_b = self._bins(th, variance)
-        o, c = self._outputs(K, nb, variance)
+        layout, buf, c = self._outputs(K, nb, variance)
         st = _capi.MiaStats()
         tu = self._tuning(tuning)
         stream = self.torch.cuda.current_stream(self.dev).cuda_stream
@@ -176,7 +191,7 @@ class Engine:
                                 METHODS[method], ctypes.byref(tu) if tu else None,
                                 ctypes.byref(st), ctypes.c_void_p(stream))
         _capi.check(self.lib, self.ctx, rc)
-        return self._host(o, K, th, st.as_dict())
+        return self._host((layout, buf), K, th, st.as_dict())
 
     def count_batch(self, samples: Sequence[SampleArrays], th: Thresholds,
                     variance: bool = False, tuning: Optional[dict] = None):
@@ -191,8 +206,8 @@ class Engine:
             keep.append(k)
             mss[i] = ms
             K = s.jk_per_side ** 3 if s.jk_per_side > 0 else 0
-            o, c = self._outputs(K, nb, variance)
-            outs.append(o)
+            layout, buf, c = self._outputs(K, nb, variance)
+            outs.append((layout, buf))
             Ks.append(K)
             cs[i] = c
         bins, keep_b = self._bins(th, variance)

@@ -55,6 +55,13 @@ def make_case(name):
         pos = wrap_box(rng.uniform(0, L, (2500, 3)), L)
         shp = wrap_box(rng.uniform(0, L, (1200, 3)), L)
         cfg.update(periodic=False, r_max=12.0, pi_max=14.0)
+    elif name == "reach_beyond_half_box":
+        # r_max > L/2: every column and slab is reached once and separations
+        # wrap on any axis (minimum image, measure_w_box_jk.py:160-162)
+        L = 10.0
+        pos = wrap_box(rng.uniform(0, L, (700, 3)), L)
+        shp = wrap_box(rng.uniform(0, L, (300, 3)), L)
+        cfg.update(r_min=0.3, r_max=7.0, nr=6, n2=6, pi_max=None, L_sub=2)
     elif name == "tiny_box_wrap":
         L = 9.0
         pos = wrap_box(rng.uniform(0, L, (800, 3)), L)
@@ -74,7 +81,7 @@ def make_case(name):
 
 
 CASES = ["uniform_jk27", "uniform_auto_nojk", "clustered_jk8", "weighted_los0", "los1_fullpi",
-         "lattice_boundaries", "tiny_box_wrap", "open_box_jk27"]
+         "lattice_boundaries", "tiny_box_wrap", "open_box_jk27", "reach_beyond_half_box"]
 
 
 def oracle_counts(data, L, cfg, mode):
