@@ -147,7 +147,9 @@ typedef struct mia_stats {
     int64_t items;          /* work items (MIA_FAST)                                   */
     int64_t rechecks;       /* pairs re-evaluated in FP64 near an edge (MIA_FAST)      */
     int32_t columns_per_side, slabs_per_side, slabs_per_block, block_shapes;
-    int32_t waves, reserved;
+    int32_t waves;
+    int32_t method;         /* counting method actually used (MIA_FAST falls back to
+                               MIA_PLAIN when its LDS histogram or key space is too big) */
 } mia_stats;
 
 int mia_abi_version(void);

@@ -57,10 +57,10 @@ class MiaStats(ctypes.Structure):
                 ("rechecks", ctypes.c_int64),
                 ("columns_per_side", ctypes.c_int32), ("slabs_per_side", ctypes.c_int32),
                 ("slabs_per_block", ctypes.c_int32), ("block_shapes", ctypes.c_int32),
-                ("waves", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("waves", ctypes.c_int32), ("method", ctypes.c_int32)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 # name -> (restype, argtypes): the complete exported surface of the header.

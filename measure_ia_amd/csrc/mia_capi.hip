@@ -122,6 +122,10 @@ int mia_count(mia_ctx *ctx, const mia_sample *sample, const mia_bins *bins, cons
         MIA_HIP(hipSetDevice(ctx->c.device));
         hipStream_t st = (hipStream_t)stream;
         if (stats) std::memset(stats, 0, sizeof(*stats));
+        // binnings or jackknife counts the batched kernel cannot hold go to the
+        // plain kernel, which has no LDS histogram or planner key space
+        if (method == MIA_FAST && !mia::fast_path_fits(*bins, weighted, sample->jk_per_side))
+            method = MIA_PLAIN;
         mia::Grid g = mia::make_grid(*sample, *bins, tuning);
         mia::Sorted pos, shp;
         int *oor = ctx->c.arena.get<int>("oor", 1);
@@ -154,6 +158,7 @@ int mia_count(mia_ctx *ctx, const mia_sample *sample, const mia_bins *bins, cons
             stats->ms_prepare = ms;
             stats->columns_per_side = g.nc;
             stats->slabs_per_side = g.nz;
+            stats->method = method;
         }
     });
 }
@@ -176,6 +181,10 @@ int mia_count_batch(mia_ctx *ctx, int32_t n, const mia_sample *samples, const mi
             check_sample(&samples[i]);
             bool weighted = samples[i].pos_w != nullptr || samples[i].shp_w != nullptr;
             check_out(&outs[i], weighted);
+            if (!mia::fast_path_fits(*bins, true, samples[i].jk_per_side))
+                throw mia::Error{MIA_E_UNSUPPORTED,
+                                 "binning or jackknife count too large for the batched kernel; "
+                                 "count the samples one by one (mia_count falls back to MIA_PLAIN)"};
             v[i].s = &samples[i];
             v[i].g = mia::make_grid(samples[i], *bins, tuning);
             v[i].out = &outs[i];
@@ -195,6 +204,7 @@ int mia_count_batch(mia_ctx *ctx, int32_t n, const mia_sample *samples, const mi
             stats->ms_prepare = ms;
             stats->columns_per_side = v[0].g.nc;
             stats->slabs_per_side = v[0].g.nz;
+            stats->method = MIA_FAST;
         }
     });
 }

@@ -148,6 +148,9 @@ struct FastSample {
     const mia_counts *out;
     bool weighted;
 };
+// Whether the batched kernel can count this binning: one wave's histogram
+// must fit in LDS and the (1+K)^2 planner keys must stay bounded.
+bool fast_path_fits(const mia_bins &bins, bool weighted, int jk);
 // oor: device out-of-box flags of the samples (checked with the first readback), or NULL
 void count_fast(Ctx &ctx, std::vector<FastSample> &samples, const mia_bins &bins,
                 const mia_tuning *tuning, hipStream_t st, mia_stats *stats, const int *oor);

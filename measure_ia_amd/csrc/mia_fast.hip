@@ -35,8 +35,10 @@ namespace mia {
 
 namespace {
 
-constexpr int kWavesPerBlock = 4;
-constexpr int kThreads = kWave * kWavesPerBlock;
+constexpr int kMaxWavesPerBlock = 4;
+constexpr int kThreads = kWave * kMaxWavesPerBlock;
+constexpr size_t kLdsBytes = 160 * 1024;         // LDS per CU (gfx950)
+constexpr int64_t kMaxKeysPerSample = 1 << 24;   // (1 + K)^2 planner keys
 constexpr double kSin2Exact = 1e-10;  // FP64 arccos-domain check below this sin^2
 
 struct Block {
@@ -658,7 +660,7 @@ __global__ __launch_bounds__(kThreads) void k_pairs(PairArgs P) {
     for (int i = lane * 4; i < P.hist_stride; i += kWave * 4) *(uint32_t *)(hist + i) = 0u;
     __syncthreads();
 
-    const int w = blockIdx.x * kWavesPerBlock + wv;
+    const int w = blockIdx.x * (blockDim.x / kWave) + wv;
     if (w >= P.n_waves || P.n_items == 0) return;
     const MIA_CONST int64_t *cpre = (const MIA_CONST int64_t *)P.cost_prefix;
     const double total_cost = (double)*(const MIA_CONST int64_t *)P.total_cost;
@@ -717,31 +719,49 @@ __global__ __launch_bounds__(kThreads) void k_pairs(PairArgs P) {
 }
 
 template <int MODE, bool WEIGHTED, bool VAR>
-void launch_pairs(const PairArgs &A, unsigned grid, size_t lds, hipStream_t st) {
+void launch_pairs(const PairArgs &A, unsigned grid, int wpb, size_t lds, hipStream_t st) {
     MIA_HIP(hipFuncSetAttribute((const void *)k_pairs<MODE, WEIGHTED, VAR>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((k_pairs<MODE, WEIGHTED, VAR>), dim3(grid), dim3(kThreads), lds, st, A);
+    hipLaunchKernelGGL((k_pairs<MODE, WEIGHTED, VAR>), dim3(grid), dim3(kWave * wpb), lds, st, A);
     MIA_HIP(hipGetLastError());
 }
 
 template <int MODE>
-void launch_pairs_mode(const PairArgs &A, unsigned grid, size_t lds, hipStream_t st) {
+void launch_pairs_mode(const PairArgs &A, unsigned grid, int wpb, size_t lds, hipStream_t st) {
     if (A.weighted) {
         if (A.var)
-            launch_pairs<MODE, true, true>(A, grid, lds, st);
+            launch_pairs<MODE, true, true>(A, grid, wpb, lds, st);
         else
-            launch_pairs<MODE, true, false>(A, grid, lds, st);
+            launch_pairs<MODE, true, false>(A, grid, wpb, lds, st);
     } else {
         if (A.var)
-            launch_pairs<MODE, false, true>(A, grid, lds, st);
+            launch_pairs<MODE, false, true>(A, grid, wpb, lds, st);
         else
-            launch_pairs<MODE, false, false>(A, grid, lds, st);
+            launch_pairs<MODE, false, false>(A, grid, wpb, lds, st);
     }
+}
+
+// LDS bytes of one wave's histogram (one copy) and of the block's thresholds.
+size_t wave_hist_bytes(const mia_bins &bins, bool weighted) {
+    const size_t B = (size_t)bins.n_r * bins.n_2;
+    size_t off = (4 * B + 15) & ~(size_t)15;
+    off += 16 * B + (weighted ? 8 * B : 0) + (bins.accumulate_variance ? 8 * B : 0);
+    return (off + 15) & ~(size_t)15;
+}
+
+size_t threshold_bytes(const mia_bins &bins) {
+    return ((size_t)(bins.n_r + 1 + bins.n_2 + 1) * 8 + 15) & ~(size_t)15;
 }
 
 inline unsigned nblk(int64_t n, int t) { return (unsigned)((n + t - 1) / t < 1 ? 1 : (n + t - 1) / t); }
 
 }  // namespace
+
+bool fast_path_fits(const mia_bins &bins, bool weighted, int jk) {
+    const int64_t K1 = jk > 0 ? (int64_t)jk * jk * jk : 1;
+    if (K1 * K1 > kMaxKeysPerSample) return false;
+    return threshold_bytes(bins) + wave_hist_bytes(bins, weighted) <= kLdsBytes;
+}
 
 void count_fast(Ctx &ctx, std::vector<FastSample> &samples, const mia_bins &bins,
                 const mia_tuning *tuning, hipStream_t st, mia_stats *stats, const int *oor) {
@@ -1006,20 +1026,24 @@ void count_fast(Ctx &ctx, std::vector<FastSample> &samples, const mia_bins &bins
     if (A.var) off += 8 * BC;
     A.hist_stride = (off + 15) & ~15;
     A.rechecks = rechecks;
-    size_t thr_bytes = ((size_t)(n_r + 1 + n_2 + 1) * 8 + 15) & ~(size_t)15;
-    size_t lds = thr_bytes + (size_t)A.hist_stride * kWavesPerBlock;
-    MIA_REQUIRE(lds <= 160 * 1024, "histogram too large for LDS (n_r*n_2 too big)");
-    int blocks_per_cu = (int)std::max<size_t>(1, std::min<size_t>(8, (160 * 1024) / lds));
-    int waves = tuning && tuning->waves > 0 ? tuning->waves
-                                            : ctx.cu_count * blocks_per_cu * kWavesPerBlock;
+    // waves per block: 4, or fewer when the histograms of 4 waves do not fit
+    const size_t thr_bytes = threshold_bytes(bins);
+    int wpb = kMaxWavesPerBlock;
+    while (wpb > 1 && thr_bytes + (size_t)A.hist_stride * wpb > kLdsBytes) wpb /= 2;
+    const size_t lds = thr_bytes + (size_t)A.hist_stride * wpb;
+    if (lds > kLdsBytes)
+        throw Error{MIA_E_UNSUPPORTED, "histogram too large for LDS (n_r*n_2 too big for MIA_FAST)"};
+    // at most 8 waves per SIMD (32 per CU)
+    int blocks_per_cu = (int)std::max<size_t>(1, std::min<size_t>(32 / wpb, kLdsBytes / lds));
+    int waves = tuning && tuning->waves > 0 ? tuning->waves : ctx.cu_count * blocks_per_cu * wpb;
     if (nitems < waves) waves = (int)std::max<int64_t>(1, nitems);
     A.n_waves = waves;
-    unsigned grid = (unsigned)((waves + kWavesPerBlock - 1) / kWavesPerBlock);
+    unsigned grid = (unsigned)((waves + wpb - 1) / wpb);
     if (nitems > 0) {
         if (bins.mode == MIA_RPPI)
-            launch_pairs_mode<MIA_RPPI>(A, grid, lds, st);
+            launch_pairs_mode<MIA_RPPI>(A, grid, wpb, lds, st);
         else
-            launch_pairs_mode<MIA_RMU>(A, grid, lds, st);
+            launch_pairs_mode<MIA_RMU>(A, grid, wpb, lds, st);
     }
     MIA_HIP(hipEventRecord(ev[2], st));
     MIA_HIP(hipEventRecord(ev[3], st));

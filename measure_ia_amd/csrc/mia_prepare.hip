@@ -208,6 +208,14 @@ __global__ void k_scatter(const double *__restrict__ xyz, int64_t n, Grid g,
             out.ay[d] = ay;
             out.e[d] = ei;
         }
+        // a shape whose axis or e is NaN (e.g. a zero axis vector normalised on
+        // the host) adds e+ = ex = 0, as the reference's isnan rule does
+        // (measure_w_box_jk.py:174-175): zero its axis and e in the record
+        if (!(ax == ax) || !(ay == ay) || !(ei == ei)) {
+            ax = 0.0;
+            ay = 0.0;
+            ei = 0.0;
+        }
         double ew = ei * wi;
         double2 *r = (double2 *)(out.rec + 8 * d);
         r[0] = make_double2(a, b);

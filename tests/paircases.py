@@ -62,6 +62,13 @@ def make_case(name):
         pos = wrap_box(rng.uniform(0, L, (700, 3)), L)
         shp = wrap_box(rng.uniform(0, L, (300, 3)), L)
         cfg.update(r_min=0.3, r_max=7.0, nr=6, n2=6, pi_max=None, L_sub=2)
+    elif name == "degenerate_axes":
+        # zero-length axis vectors normalise to NaN; their pairs count in DD
+        # with e+ = ex = 0 (the reference's isnan rule, measure_w_box_jk.py:174-175)
+        L = 40.0
+        pos = wrap_box(rng.uniform(0, L, (1500, 3)), L)
+        shp = wrap_box(rng.uniform(0, L, (700, 3)), L)
+        cfg.update(r_max=8.0, pi_max=10.0, L_sub=2)
     elif name == "tiny_box_wrap":
         L = 9.0
         pos = wrap_box(rng.uniform(0, L, (800, 3)), L)
@@ -72,6 +79,8 @@ def make_case(name):
     e12 = ellipticity_noise(rng, len(shp), 0.25)
     axis, q = shapes_from_components(e12)
     axis = axis * rng.uniform(0.5, 2.0, (len(shp), 1))  # unnormalised on input
+    if name == "degenerate_axes":
+        axis[rng.choice(len(shp), len(shp) // 20, replace=False)] = 0.0
     data = {"Position": pos, "Position_shape_sample": shp, "Axis_Direction": axis,
             "LOS": cfg["los"], "q": q}
     if cfg["weights"]:
@@ -81,7 +90,8 @@ def make_case(name):
 
 
 CASES = ["uniform_jk27", "uniform_auto_nojk", "clustered_jk8", "weighted_los0", "los1_fullpi",
-         "lattice_boundaries", "tiny_box_wrap", "open_box_jk27", "reach_beyond_half_box"]
+         "lattice_boundaries", "tiny_box_wrap", "open_box_jk27", "reach_beyond_half_box",
+         "degenerate_axes"]
 
 
 def oracle_counts(data, L, cfg, mode):
