@@ -38,7 +38,7 @@ namespace {
 constexpr int kMaxWavesPerBlock = 4;
 constexpr int kThreads = kWave * kMaxWavesPerBlock;
 constexpr size_t kLdsBytes = 160 * 1024;         // LDS per CU (gfx950)
-constexpr int64_t kMaxKeysPerSample = 1 << 24;   // (1 + K)^2 planner keys
+constexpr int64_t kMaxKeysPerSample = 1 << 24;   // K^2 planner keys per sample (K regions)
 constexpr double kSin2Exact = 1e-10;  // FP64 arccos-domain check below this sin^2
 
 struct Block {
