@@ -174,6 +174,57 @@ int mia_count_batch(mia_ctx *ctx, int32_t n, const mia_sample *samples, const mi
                     const mia_counts *outs, const mia_tuning *tuning, mia_stats *stats,
                     void *stream);
 
+/* One lightcone measurement: a position sample and a shape sample on the sky
+ * (RA, DEC in degrees, comoving distance chi), as the reference's lightcone
+ * counters prepare them (src/measureia/measure_w_lightcone.py:82-139,
+ * measure_m_lightcone.py:82-139).  All pointers are device pointers.
+ * The host supplies chi already multiplied by h when over_h, cos(dec/180*pi)
+ * of every position (numpy's value), and per shape the rotated ellipticity
+ *   c2 = -e cos(2 phi_axis),  s2 = -e sin(2 phi_axis)
+ * (e = |e1 + i e2|, phi_axis the axis angle the reference derives from e1, e2),
+ * so that e+ = c2 cos 2phi_sep + s2 sin 2phi_sep, ex = s2 cos 2phi_sep - c2 sin 2phi_sep. */
+typedef struct mia_lc_sample {
+    const double *ra, *dec, *chi;  /* positions, n_pos each                                  */
+    const double *cos_dec;         /* n_pos: cos(dec / 180 * pi)                              */
+    const double *w;               /* n_pos weights, or NULL for unit weights                 */
+    const int32_t *patch;          /* n_pos jackknife patch in [0, n_patches), or NULL        */
+    const double *ra_s, *dec_s, *chi_s; /* shape sample, n_shape each                         */
+    const double *w_s;             /* n_shape weights, or NULL                                */
+    const double *c2, *s2;         /* n_shape rotated ellipticity; both NULL: pair counts only */
+    const int32_t *patch_s;        /* n_shape patch, or NULL                                  */
+    int64_t n_pos;
+    int64_t n_shape;
+    double proj_scale;             /* h with over_h, else 1: multiplies (dx, dy) before r_p    */
+    double rp2_cut;                /* MIA_RMU: pairs need fl(dx'^2 + dy'^2) > rp2_cut (the
+                                      squared-separation threshold of rp_cut)                 */
+    int32_t n_patches;             /* 0: no jackknife tallies; at most 4096                    */
+    int32_t reserved;
+} mia_lc_sample;
+
+/* Count one lightcone measurement: replaces the per-position loops of
+ * _measure_xi_rp_pi_lightcone_brute / _count_pairs_xi_rp_pi_lightcone_brute
+ * (measure_w_lightcone.py:45-343), _measure_xi_r_mur_lightcone_brute /
+ * _count_pairs_xi_r_mur_lightcone_brute (measure_m_lightcone.py:45-360), and
+ * the per-patch reruns of _measure_jackknife_realisations_lightcone[_multiprocessing]
+ * (measure_jackknife.py:59-171, 265-483).  Per ordered pair (shape s, position n):
+ *   pi = chi_s - chi_n
+ *   dx = ((ra_s - ra_n)/180*pi)*chi_n*cos_dec_n,  dy = ((dec_s - dec_n)/180*pi)*chi_n
+ *   MIA_RPPI: s = fl((h dx)^2 + (h dy)^2) binned on bins.r2_edges, pi on x2_edges
+ *   MIA_RMU : r^2 = fl(fl(dx^2 + dy^2) + pi^2), mu = pi/r, and fl((h dx)^2+(h dy)^2) > rp2_cut
+ * Outputs (device, zeroed and filled), K = n_patches, B = n_r*n_2:
+ *   npairs [(1+K)*B] integer pair counts, [0] all pairs, [1+i] pairs touching patch i
+ *   dd     [(1+K)*B] the same tallies of w_n*w_s (required)
+ *   splus  [(1+K)*B] tallies of w_n*w_s*e+   (required with c2/s2, else ignored)
+ *   scross [(1+K)*B] tallies of w_n*w_s*ex   (required with c2/s2, else ignored)
+ * Realisation i (the sample without patch i) is total - tally[i].  Tuning
+ * (optional): slabs_per_side / columns_per_side = DEC rows / RA columns of
+ * the angular grid, cell_shapes = target shapes per cell, slabs_per_block =
+ * depth of a position block in eighths of the chi window, block_shapes =
+ * positions per block (<= 64), item_positions = shapes per work item, waves. */
+int mia_count_lightcone(mia_ctx *ctx, const mia_lc_sample *sample, const mia_bins *bins,
+                        const mia_counts *out, const mia_tuning *tuning, mia_stats *stats,
+                        void *stream);
+
 /* Jackknife region id of each object (measure_IA_base.py:404-452): strict
  * i*L_sub < x < (i+1)*L_sub on every axis, region i*L^2 + j*L + k, else 0.
  * xyz device n x 3, region device n int32. */

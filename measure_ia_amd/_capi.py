@@ -31,6 +31,17 @@ class MiaSample(ctypes.Structure):
                 ("jk_per_side", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
+class MiaLcSample(ctypes.Structure):
+    _fields_ = [("ra", ctypes.c_void_p), ("dec", ctypes.c_void_p), ("chi", ctypes.c_void_p),
+                ("cos_dec", ctypes.c_void_p), ("w", ctypes.c_void_p), ("patch", ctypes.c_void_p),
+                ("ra_s", ctypes.c_void_p), ("dec_s", ctypes.c_void_p), ("chi_s", ctypes.c_void_p),
+                ("w_s", ctypes.c_void_p), ("c2", ctypes.c_void_p), ("s2", ctypes.c_void_p),
+                ("patch_s", ctypes.c_void_p),
+                ("n_pos", ctypes.c_int64), ("n_shape", ctypes.c_int64),
+                ("proj_scale", ctypes.c_double), ("rp2_cut", ctypes.c_double),
+                ("n_patches", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
 class MiaBins(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int32), ("n_r", ctypes.c_int32), ("n_2", ctypes.c_int32),
                 ("accumulate_variance", ctypes.c_int32),
@@ -77,6 +88,10 @@ SIGNATURES = {
                                        ctypes.POINTER(MiaBins), ctypes.POINTER(MiaCounts),
                                        ctypes.POINTER(MiaTuning), ctypes.POINTER(MiaStats),
                                        ctypes.c_void_p]),
+    "mia_count_lightcone": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(MiaLcSample),
+                                           ctypes.POINTER(MiaBins), ctypes.POINTER(MiaCounts),
+                                           ctypes.POINTER(MiaTuning), ctypes.POINTER(MiaStats),
+                                           ctypes.c_void_p]),
     "mia_region_ids": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
                                       ctypes.c_double, ctypes.c_int32, ctypes.c_void_p,
                                       ctypes.c_void_p]),

@@ -209,6 +209,34 @@ int mia_count_batch(mia_ctx *ctx, int32_t n, const mia_sample *samples, const mi
     });
 }
 
+int mia_count_lightcone(mia_ctx *ctx, const mia_lc_sample *s, const mia_bins *bins,
+                        const mia_counts *out, const mia_tuning *tuning, mia_stats *stats,
+                        void *stream) {
+    return guard(ctx, [&] {
+        MIA_REQUIRE(s != nullptr, "sample is NULL");
+        check_bins(bins);
+        MIA_REQUIRE(s->n_pos >= 0 && s->n_shape >= 0, "negative sample size");
+        MIA_REQUIRE(s->n_pos < (1ll << 31) && s->n_shape < (1ll << 31), "sample larger than 2^31 objects");
+        MIA_REQUIRE(s->n_pos == 0 || (s->ra && s->dec && s->chi && s->cos_dec), "position arrays are NULL");
+        MIA_REQUIRE(s->n_shape == 0 || (s->ra_s && s->dec_s && s->chi_s), "shape arrays are NULL");
+        MIA_REQUIRE((s->c2 == nullptr) == (s->s2 == nullptr), "c2 and s2 must both be given or both NULL");
+        MIA_REQUIRE(s->n_patches >= 0 && s->n_patches <= 4096, "n_patches out of range");
+        MIA_REQUIRE(s->n_patches == 0 || s->n_pos == 0 || s->patch, "patch is NULL");
+        MIA_REQUIRE(s->n_patches == 0 || s->n_shape == 0 || s->patch_s, "patch_s is NULL");
+        MIA_REQUIRE(s->proj_scale > 0 && std::isfinite(s->proj_scale), "proj_scale must be positive");
+        MIA_REQUIRE(!std::isnan(s->rp2_cut), "rp2_cut is NaN");
+        MIA_REQUIRE(out && out->npairs && out->dd, "npairs and dd are required");
+        MIA_REQUIRE(s->c2 == nullptr || (out->splus && out->scross),
+                    "splus and scross are required with shapes");
+        if (!mia::lightcone_fits(*bins))
+            throw mia::Error{MIA_E_UNSUPPORTED, "binning too large for the lightcone kernel"};
+        MIA_HIP(hipSetDevice(ctx->c.device));
+        hipStream_t st = (hipStream_t)stream;
+        mia::count_lightcone(ctx->c, *s, *bins, *out, tuning, st, stats);
+        MIA_HIP(hipStreamSynchronize(st));
+    });
+}
+
 int mia_region_ids(mia_ctx *ctx, const double *xyz, int64_t n, double jk_sub, int32_t jk_per_side,
                    int32_t *region, void *stream) {
     return guard(ctx, [&] {

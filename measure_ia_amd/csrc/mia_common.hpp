@@ -155,6 +155,13 @@ bool fast_path_fits(const mia_bins &bins, bool weighted, int jk);
 void count_fast(Ctx &ctx, std::vector<FastSample> &samples, const mia_bins &bins,
                 const mia_tuning *tuning, hipStream_t st, mia_stats *stats, const int *oor);
 
+// Lightcone counter (mia_lightcone.hip): whether the binning's histogram fits
+// one wave's LDS share, and the count itself (outputs zeroed and filled).
+bool lightcone_fits(const mia_bins &bins);
+struct Ctx;
+void count_lightcone(Ctx &ctx, const mia_lc_sample &s, const mia_bins &bins, const mia_counts &out,
+                     const mia_tuning *tuning, hipStream_t st, mia_stats *stats);
+
 struct Ctx {
     int device = 0;
     std::string last_error;

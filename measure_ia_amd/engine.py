@@ -35,6 +35,31 @@ class SampleArrays:
 
 
 @dataclass
+class LightconeArrays:
+    """One lightcone measurement as the lightcone counters prepare it (host or device arrays).
+
+    chi is the comoving distance (already times h with over_h); cos_dec is
+    cos(dec/180*pi) of every position; c2, s2 are -e cos 2phi_axis and
+    -e sin 2phi_axis of every shape (None for pair counts only)."""
+    ra: object
+    dec: object
+    chi: object
+    cos_dec: object
+    ra_s: object
+    dec_s: object
+    chi_s: object
+    weight: Optional[object] = None
+    weight_shape: Optional[object] = None
+    c2: Optional[object] = None
+    s2: Optional[object] = None
+    patch: Optional[object] = None         # int patch index in [0, n_patches)
+    patch_s: Optional[object] = None
+    n_patches: int = 0
+    proj_scale: float = 1.0
+    rp2_cut: float = 0.0
+
+
+@dataclass
 class CountResult:
     npairs: np.ndarray           # (1+K, n_r, n_2) int64 -- [0] total, [1+i] tally of region i
     dd: np.ndarray               # (1+K, n_r, n_2) float64
@@ -220,6 +245,70 @@ class Engine:
         _capi.check(self.lib, self.ctx, rc)
         stats = st.as_dict()
         return [self._host(o, K, th, stats) for o, K in zip(outs, Ks)]
+
+    def _dev_i32(self, a):
+        t = self.torch
+        if a is None:
+            return None
+        if isinstance(a, t.Tensor):
+            return a.to(device=self.dev, dtype=t.int32).contiguous()
+        return t.from_numpy(np.ascontiguousarray(np.asarray(a, dtype=np.int32))).to(self.dev)
+
+    def count_lightcone(self, lc: LightconeArrays, th: Thresholds,
+                        tuning: Optional[dict] = None) -> CountResult:
+        """Count one lightcone measurement; returns host arrays with K = n_patches
+        tallies (scross is tallied like splus)."""
+        n_p = int(np.shape(lc.ra)[0])
+        n_s = int(np.shape(lc.ra_s)[0])
+        shapes = lc.c2 is not None
+        arrs = {k: self._dev(getattr(lc, k), (n_p,)) for k in ("ra", "dec", "chi", "cos_dec")}
+        arrs.update({k: self._dev(getattr(lc, k), (n_s,)) for k in ("ra_s", "dec_s", "chi_s")})
+        arrs["w"] = None if _unit(lc.weight) else self._dev(lc.weight, (n_p,))
+        arrs["w_s"] = None if _unit(lc.weight_shape) else self._dev(lc.weight_shape, (n_s,))
+        arrs["c2"] = self._dev(lc.c2, (n_s,)) if shapes else None
+        arrs["s2"] = self._dev(lc.s2, (n_s,)) if shapes else None
+        K = int(lc.n_patches)
+        arrs["patch"] = self._dev_i32(lc.patch) if K else None
+        arrs["patch_s"] = self._dev_i32(lc.patch_s) if K else None
+
+        def ptr(x):
+            return None if x is None else x.data_ptr()
+
+        ms = _capi.MiaLcSample(n_pos=n_p, n_shape=n_s, proj_scale=float(lc.proj_scale),
+                               rp2_cut=float(lc.rp2_cut), n_patches=K,
+                               **{k: ptr(v) for k, v in arrs.items()})
+        nb = th.n_r * th.n_2
+        bins, keep_b = self._bins(th, False)
+        tt = self.torch
+        sizes = [("npairs", (1 + K) * nb), ("dd", (1 + K) * nb),
+                 ("splus", (1 + K) * nb if shapes else 0), ("scross", (1 + K) * nb if shapes else 0)]
+        buf = tt.empty(max(sum(n for _, n in sizes), 1), dtype=tt.float64, device=self.dev)
+        layout, off, ptrs = {}, 0, {}
+        for name, n in sizes:
+            layout[name] = (off, n) if n else None
+            ptrs[name] = buf.data_ptr() + 8 * off if n else None
+            off += n
+        c = _capi.MiaCounts(**ptrs)
+        st = _capi.MiaStats()
+        tu = self._tuning(tuning)
+        stream = self.torch.cuda.current_stream(self.dev).cuda_stream
+        rc = self.lib.mia_count_lightcone(self.ctx, ctypes.byref(ms), ctypes.byref(bins),
+                                          ctypes.byref(c), ctypes.byref(tu) if tu else None,
+                                          ctypes.byref(st), ctypes.c_void_p(stream))
+        _capi.check(self.lib, self.ctx, rc)
+        host = buf.cpu().numpy()
+        shp3 = (1 + K, th.n_r, th.n_2)
+
+        def part(name, dtype=None):
+            if layout[name] is None:
+                return None
+            o, n = layout[name]
+            a = host[o:o + n]
+            return (a.view(dtype) if dtype else a).reshape(shp3).copy()
+
+        return CountResult(npairs=part("npairs", np.int64), dd=part("dd"), splus=part("splus"),
+                           scross=part("scross"), variance=None, region_stats=None,
+                           stats=st.as_dict())
 
     def region_ids(self, xyz, jk_per_side: int, jk_sub: float) -> np.ndarray:
         x = self._dev(xyz)

@@ -160,3 +160,71 @@ def uneven_samples(n_samples=12, parent=3_000_000, L=205.0, lo=10_000, hi=800_00
                 "weight": rng.uniform(0.5, 1.5, n), "weight_shape_sample": rng.uniform(0.5, 1.5, ns)}
         out.append(Catalogue(data, L))
     return out
+
+
+# ---------------------------------------------------------------------------
+# lightcones
+
+def _sky_to_cart(ra, dec, chi):
+    r, d = np.radians(ra), np.radians(dec)
+    return np.stack([chi * np.cos(d) * np.cos(r), chi * np.cos(d) * np.sin(r), chi * np.sin(d)], 1)
+
+
+def _cart_to_sky(x):
+    chi = np.sqrt(np.sum(x * x, axis=1))
+    dec = np.degrees(np.arcsin(np.clip(x[:, 2] / chi, -1.0, 1.0)))
+    ra = np.degrees(np.arctan2(x[:, 1], x[:, 0])) % 360.0
+    return ra, dec, chi
+
+
+def lightcone_points(rng, n, ra=(10.0, 40.0), dec=(-10.0, 10.0), chi=(200.0, 1400.0),
+                     clustered_frac=0.5, n_clusters=None, sigma=1.5):
+    """n objects in a lightcone wedge, uniform in comoving volume, a fraction
+    of them in Gaussian clusters (comoving sigma) around uniform centres.
+    Returns ra, dec (degrees) and comoving distance chi."""
+    def uniform_pts(m):
+        r = rng.uniform(ra[0], ra[1], m)
+        s0, s1 = np.sin(np.radians(dec[0])), np.sin(np.radians(dec[1]))
+        d = np.degrees(np.arcsin(rng.uniform(s0, s1, m)))
+        c = np.cbrt(rng.uniform(chi[0] ** 3, chi[1] ** 3, m))
+        return r, d, c
+
+    n_cl = int(round(n * clustered_frac))
+    r, d, c = uniform_pts(n - n_cl)
+    if n_cl > 0:
+        k = n_clusters or max(1, n_cl // 50)
+        cr, cd, cc = uniform_pts(k)
+        centres = _sky_to_cart(cr, cd, cc)
+        members = rng.integers(0, k, n_cl)
+        x = centres[members] + rng.normal(0.0, sigma, (n_cl, 3))
+        mr, md, mc = _cart_to_sky(x)
+        # keep members inside the wedge's chi range
+        mc = np.clip(mc, chi[0], chi[1])
+        r, d, c = np.concatenate([r, mr]), np.concatenate([d, md]), np.concatenate([c, mc])
+    p = rng.permutation(n)
+    return r[p], d[p], c[p]
+
+
+def lightcone(n_pos, n_shape, seed=0, sigma_e=0.25, weights=False, **kw):
+    """Position and shape samples of one lightcone wedge (shapes a random
+    subset of the positions) with ellipticity components e1, e2.  Returns a
+    dict of arrays keyed like the reference's lightcone data dict, with the
+    comoving distances under "chi" / "chi_shape_sample" (Redshift is filled
+    by the caller from a cosmology when needed)."""
+    rng = np.random.default_rng(seed)
+    ra, dec, chi = lightcone_points(rng, n_pos, **kw)
+    sel = rng.choice(n_pos, n_shape, replace=n_shape > n_pos)
+    e12 = ellipticity_noise(rng, n_shape, sigma_e)
+    out = {"RA": ra, "DEC": dec, "chi": chi, "RA_shape_sample": ra[sel], "DEC_shape_sample": dec[sel],
+           "chi_shape_sample": chi[sel], "e1": e12[:, 0], "e2": e12[:, 1]}
+    if weights:
+        out["weight"] = rng.uniform(0.5, 1.5, n_pos)
+        out["weight_shape_sample"] = rng.uniform(0.5, 1.5, n_shape)
+    return out
+
+
+def redshift_of_chi(cosmology, chi):
+    """Inverse of the cosmology's chi(a): z with chi(1/(1+z)) = chi (table inversion)."""
+    z = np.linspace(0.0, 10.0, 200001)
+    c = np.asarray(cosmology.comoving_radial_distance(1.0 / (1.0 + z)))
+    return np.interp(np.asarray(chi), c, z)

@@ -47,7 +47,7 @@ def test_library_exports_every_symbol(lib):
 def test_struct_layouts_match_header(tmp_path):
     structs = {"mia_sample": _capi.MiaSample, "mia_bins": _capi.MiaBins,
                "mia_counts": _capi.MiaCounts, "mia_tuning": _capi.MiaTuning,
-               "mia_stats": _capi.MiaStats}
+               "mia_stats": _capi.MiaStats, "mia_lc_sample": _capi.MiaLcSample}
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){"]
     for s, cls in structs.items():
         lines.append(f'printf("{s} %zu\\n", sizeof({s}));')
@@ -70,3 +70,4 @@ def test_null_context_is_rejected(lib):
     rc = lib.mia_count(None, None, None, None, 0, None, None, None)
     assert rc == -1
     assert lib.mia_region_ids(None, None, 0, 1.0, 1, None, None) == -1
+    assert lib.mia_count_lightcone(None, None, None, None, None, None, None) == -1
