@@ -1,0 +1,129 @@
+"""MeasureIALightcone end to end against expected outputs built from the oracle.
+
+One small lightcone (1,200 positions, 600 shapes, 3,000 randoms, 4 RA-band
+jackknife patches) through measure_xi_w and measure_xi_multipoles with both
+estimators; the HDF5 datasets -- counts, estimators, projections, every
+realisation's counts and the jackknife covariance -- are compared with
+tests/reflc.py.  Pair counts of unit-weight samples must be equal; every
+other dataset within 1e-6 of its largest value.
+"""
+import numpy as np
+import pytest
+
+from measure_ia_amd import synth
+from measure_ia_amd.hdf5io import H5Store
+from measure_ia_amd.lightcone import MeasureIALightcone, comoving_distance, default_cosmology
+from reflc import expected
+
+pytestmark = pytest.mark.gpu
+
+P = 4
+KW = dict(ra=(10.0, 22.0), dec=(-4.0, 4.0), chi=(300.0, 800.0))
+
+
+def _catalogue():
+    cosmo = default_cosmology()
+    a = synth.lightcone(1200, 600, seed=21, weights=True, **KW)
+    rng = np.random.default_rng(22)
+    rr, rd, rc = synth.lightcone_points(rng, 3000, clustered_frac=0.0, **KW)
+    data = {"RA": a["RA"], "DEC": a["DEC"], "Redshift": synth.redshift_of_chi(cosmo, a["chi"]),
+            "RA_shape_sample": a["RA_shape_sample"], "DEC_shape_sample": a["DEC_shape_sample"],
+            "Redshift_shape_sample": synth.redshift_of_chi(cosmo, a["chi_shape_sample"]),
+            "e1": a["e1"], "e2": a["e2"], "weight": a["weight"], "weight_shape_sample": a["weight_shape_sample"]}
+    randoms = {"RA": rr, "DEC": rd, "Redshift": synth.redshift_of_chi(cosmo, rc)}
+
+    def band(ra):
+        return np.clip(((ra - KW["ra"][0]) / (KW["ra"][1] - KW["ra"][0]) * P).astype(int), 0, P - 1)
+
+    patches = {"position": band(data["RA"]), "shape": band(data["RA_shape_sample"]), "randoms": band(rr)}
+    return data, randoms, patches
+
+
+def _read(fn):
+    with H5Store(fn, "r") as f:
+        return {p: f.read(p) for p in f.walk("")}
+
+
+def _close(got, ref, what, exact=False):
+    got = np.asarray(got, dtype=np.float64)
+    ref = np.asarray(ref, dtype=np.float64)
+    assert got.shape == ref.shape, what
+    if exact:
+        np.testing.assert_array_equal(got, ref, err_msg=what)
+        return
+    m = np.abs(ref).max()
+    assert np.abs(got - ref).max() <= 1e-6 * max(m, 1e-300), what
+
+
+@pytest.mark.parametrize("group", ["w", "multipoles"])
+@pytest.mark.parametrize("est", ["galaxies", "clusters"])
+def test_lightcone_frontend(tmp_path, group, est):
+    data, randoms, patches = _catalogue()
+    fn = str(tmp_path / "lc.hdf5")
+    lc = MeasureIALightcone(data, randoms, fn, separation_limits=[0.5, 15.0], num_bins_r=8, num_bins_pi=10,
+                            pi_max=20.0)
+    jk = {k: v.copy() for k, v in patches.items()}
+    if group == "w":
+        lc.measure_xi_w(est, "gp", "both", jk_patches=jk)
+        mode, second = "rppi", lc.pi_bins
+    else:
+        lc.measure_xi_multipoles(est, "gp", "both", jk_patches=jk, over_h=False)
+        mode, second = "rmu", lc.mu_r_bins
+    got = _read(fn)
+    cosmo = default_cosmology()
+    chi = {"D": comoving_distance(cosmo, 1 / (1 + data["Redshift"])),
+           "S": comoving_distance(cosmo, 1 / (1 + data["Redshift_shape_sample"])),
+           "R": comoving_distance(cosmo, 1 / (1 + randoms["Redshift"]))}
+    D = {"RA": data["RA"], "DEC": data["DEC"], "w": data["weight"]}
+    S = {"RA": data["RA_shape_sample"], "DEC": data["DEC_shape_sample"], "w": data["weight_shape_sample"],
+         "e1": data["e1"], "e2": data["e2"]}
+    R = {"RA": randoms["RA"], "DEC": randoms["DEC"], "w": np.ones(len(randoms["RA"]))}
+    ref = expected(mode, est, D, S, R, chi, lc.r_bins, second, 0.5, 15.0, patches)
+    g = group
+    m = ref["main"]
+    assert m["RR"].sum() > 1000 and np.abs(m["SpD"]).sum() > 0
+    _close(got[f"{g}/xi_g_plus/gp_SplusD"], m["SpD"], "S+D")
+    _close(got[f"{g}/xi_g_cross/gp_ScrossD"], m["SxD"], "SxD")
+    _close(got[f"{g}/xi_gg/gp_DD"], m["DD"], "DD")
+    _close(got[f"{g}/xi_g_plus/gp_randoms_SplusD"], m["SpR"], "S+R")
+    _close(got[f"{g}/xi_gg/gp_randoms_DD"], m["SR"], "SR")
+    _close(got[f"{g}/xi_gg/gp_RD"], m["RD"], "RD")
+    _close(got[f"{g}/xi_gg/gp_RR"], m["RR"], "RR", exact=True)
+    _close(got[f"{g}/xi_g_plus/gp"], ref["xi_g_plus"], "xi_g+")
+    _close(got[f"{g}/xi_gg/gp"], ref["xi_gg"], "xi_gg")
+    proj = "w" if g == "w" else "multipoles"
+    _close(got[f"{proj}_g_plus/gp"], ref[f"{proj}_g_plus"], "projection g+")
+    _close(got[f"{proj}_gg/gp"], ref[f"{proj}_gg"], "projection gg")
+    for i, (c, _, _) in ref["reals"].items():
+        _close(got[f"{g}/xi_g_plus/gp_jk{P}/gp_{i}_SplusD"], c["SpD"], f"S+D real {i}")
+        _close(got[f"{g}/xi_gg/gp_jk{P}/gp_{i}_DD"], c["DD"], f"DD real {i}")
+        _close(got[f"{g}/xi_gg/gp_jk{P}/gp_{i}_RR"], c["RR"], f"RR real {i}", exact=True)
+        _close(got[f"{g}/xi_gg/gp_jk{P}/gp_{i}_RD"], c["RD"], f"RD real {i}")
+        _close(got[f"{g}/xi_g_plus/gp_randoms_jk{P}/gp_randoms_{i}_SplusD"], c["SpR"], f"S+R real {i}")
+    for name in ("g_plus", "gg"):
+        _close(got[f"{proj}_{name}/gp_mean_{P}"], ref[f"mean_{name}"], f"mean {name}")
+        _close(got[f"{proj}_{name}/gp_jackknife_cov_{P}"], ref[f"cov_{name}"], f"cov {name}")
+
+
+def test_lightcone_pooled_layout(tmp_path):
+    """num_nodes > 1 writes the pooled loop's realisation layout
+    (measure_jackknife.py:436-481): only S+D / DD (and the pair counts)."""
+    data, randoms, patches = _catalogue()
+    fn = str(tmp_path / "lc.hdf5")
+    lc = MeasureIALightcone(data, randoms, fn, separation_limits=[0.5, 15.0], num_bins_r=8, num_bins_pi=10,
+                            pi_max=20.0, num_nodes=4)
+    lc.measure_xi_w("galaxies", "gp", "both", jk_patches={k: v.copy() for k, v in patches.items()})
+    got = _read(fn)
+    assert "w/xi_g_plus/gp_jk4/gp_0_SplusD" in got and "w/xi_gg/gp_jk4/gp_0_DD" in got
+    assert "w/xi_g_cross/gp_jk4/gp_0_ScrossD" not in got
+    assert "w/xi_g_plus/gp_jk4/gp_0_rp" in got and "w/xi_gg/gp_jk4/gp_0_RR" in got
+    assert "w_g_plus/gp_jackknife_cov_4" in got
+
+
+def test_lightcone_requires_patches_or_kmeans(tmp_path):
+    data, randoms, _ = _catalogue()
+    lc = MeasureIALightcone(data, randoms, str(tmp_path / "x.hdf5"), pi_max=20.0)
+    with pytest.raises((ImportError, ValueError)):
+        lc.measure_xi_w("galaxies", "gp", "both", num_jk=4)
+    with pytest.raises(ValueError):
+        MeasureIALightcone(data, randoms, None)  # neither pi_max nor a boxsize
