@@ -19,6 +19,17 @@ Workloads (synthetic, fixed seeds, built by measure_ia_amd/synth.py):
   large20M       L = 500, 20,000,000 positions (20,000 clusters, sigma = 1.5,
       30 % background) / 5,000,000 shapes (A = 0.05, noise N(0, 0.25));
       15 log r_p bins over 0.1-100, 40 pi bins over |pi| < 100, 125 regions.
+  lightcone1M    a lightcone wedge RA 0-60, DEC -15..15 deg, comoving distance
+      300-2000 Mpc: 1,000,000 positions (half in Gaussian clusters of sigma 1.5)
+      and a 300,000-galaxy shape subset with noise ellipticities, weights
+      U(0.5, 1.5); the S+D count of measure_xi_w on lightcones (10 log r_p
+      bins over 0.1-20, 20 pi bins over |pi| < 60) with 27 RA x DEC jackknife
+      patches (mia_count_lightcone).  Its CPU baseline is the oracle's
+      restatement of the reference's per-position brute loop
+      (measure_w_lightcone.py:141-192) on a seeded sample of positions.
+  lightcone_rr4M the same wedge filled with 4,000,000 uniform randoms counted
+      against themselves (the RR pair count of the lightcone estimators,
+      _count_pairs_xi_rp_pi_lightcone_brute), unit weights, 27 patches.
 
 A step is one full jackknife count of the workload (region ids, cell sort,
 work plan, pair kernel, histogram readback) with the inputs already in HBM.
@@ -66,7 +77,19 @@ WORKLOADS = {
                      pi_max=100.0, jk=5,
                      desc="20,000,000 clustered positions / 5,000,000 shapes, L=500, 125 JK, "
                           "15 log r_p bins 0.1-100, 40 pi bins |pi|<100"),
+    "lightcone1M": dict(kind="lightcone", mode="rppi", r=(0.1, 20.0), nr=10, n2=20, pi_max=60.0, jk=0,
+                        patches=(9, 3), n_pos=1_000_000, n_shape=300_000,
+                        sky=dict(ra=(0.0, 60.0), dec=(-15.0, 15.0), chi=(300.0, 2000.0)),
+                        desc="lightcone wedge RA 0-60, DEC -15..15, chi 300-2000 Mpc: 1,000,000 positions / "
+                             "300,000 shapes, weighted, 27 patches, 10 log r_p bins 0.1-20, 20 pi bins |pi|<60"),
+    "lightcone_rr4M": dict(kind="lightcone", mode="rppi", r=(0.1, 20.0), nr=10, n2=20, pi_max=60.0, jk=0,
+                           patches=(9, 3), n_pos=4_000_000, n_shape=4_000_000, randoms=True,
+                           sky=dict(ra=(0.0, 60.0), dec=(-15.0, 15.0), chi=(300.0, 2000.0)),
+                           desc="lightcone wedge RA 0-60, DEC -15..15, chi 300-2000 Mpc: 4,000,000 uniform "
+                                "randoms x themselves (RR pair count), 27 patches, 10 log r_p bins 0.1-20, "
+                                "20 pi bins |pi|<60"),
 }
+FLOP_PER_PAIR_LC = 30   # lightcone (r_p, pi) pair: see the lightcone section of the design notes
 
 
 def log(*a):
@@ -94,9 +117,9 @@ def build_catalogues(w):
     return [synth.config(w["cat"])]
 
 
-def measured_traffic(workload):
+def measured_traffic(workload, kernel="k_pairs"):
     """HBM bytes per pair-kernel launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes."""
-    fn = os.path.join(ROOT, "profiles", f"k_pairs_{workload}_traffic.json")
+    fn = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_traffic.json")
     if not os.path.exists(fn):
         return None
     with open(fn) as f:
@@ -133,6 +156,109 @@ def cpu_baseline(cat, w, r_bins, sb, pairs_per_shape):
                 seconds_full_estimate=out["seconds"] * n_chunks / len(pick))
 
 
+def lightcone_inputs(w):
+    """The workload's arrays (host) and their patch labels."""
+    from measure_ia_amd import synth
+    from measure_ia_amd.engine import LightconeArrays
+    from measure_ia_amd.lightcone import rotated_ellipticity
+    if w.get("randoms"):
+        ra, dec, chi = synth.lightcone_points(np.random.default_rng(13), w["n_pos"], clustered_frac=0.0,
+                                              **w["sky"])
+        s = {"RA": ra, "DEC": dec, "chi": chi, "RA_shape_sample": ra, "DEC_shape_sample": dec,
+             "chi_shape_sample": chi, "weight": np.ones(len(ra)), "weight_shape_sample": np.ones(len(ra))}
+    else:
+        s = synth.lightcone(w["n_pos"], w["n_shape"], seed=11, weights=True, **w["sky"])
+    nra, ndec = w["patches"]
+
+    def patch(ra, dec):
+        i = np.clip(((ra - w["sky"]["ra"][0]) / (w["sky"]["ra"][1] - w["sky"]["ra"][0]) * nra).astype(int), 0, nra - 1)
+        j = np.clip(((dec - w["sky"]["dec"][0]) / (w["sky"]["dec"][1] - w["sky"]["dec"][0]) * ndec).astype(int),
+                    0, ndec - 1)
+        return (i * ndec + j).astype(np.int32)
+
+    c2, s2 = rotated_ellipticity(s["e1"], s["e2"]) if "e1" in s else (None, None)
+    lc = LightconeArrays(ra=s["RA"], dec=s["DEC"], chi=s["chi"], cos_dec=np.cos(s["DEC"] / 180 * np.pi),
+                         ra_s=s["RA_shape_sample"], dec_s=s["DEC_shape_sample"], chi_s=s["chi_shape_sample"],
+                         weight=s["weight"], weight_shape=s["weight_shape_sample"], c2=c2, s2=s2,
+                         patch=patch(s["RA"], s["DEC"]),
+                         patch_s=patch(s["RA_shape_sample"], s["DEC_shape_sample"]), n_patches=nra * ndec)
+    return s, lc
+
+
+def lightcone_cpu_baseline(s, w, r_bins, sb, n_sample):
+    """The oracle's per-position brute loop (the reference's lightcone algorithm,
+    one core) on a seeded sample of positions against every shape."""
+    from oracle import lightcone as olc
+    rng = np.random.default_rng(12345)
+    pick = np.sort(rng.choice(len(s["RA"]), n_sample, replace=False))
+    t = time.perf_counter()
+    out = olc.count(w["mode"], s["RA"][pick], s["DEC"][pick], s["chi"][pick], s["weight"][pick],
+                    s["RA_shape_sample"], s["DEC_shape_sample"], s["chi_shape_sample"],
+                    s["weight_shape_sample"], r_bins, sb, w["r"][0], w["r"][1], e1=s.get("e1"),
+                    e2=s.get("e2"))
+    sec = time.perf_counter() - t
+    n = int(out["NP"][0].sum())
+    return dict(value=n / sec, unit="pairs/s", cores=1, kind="port",
+                sample=f"{n_sample} seeded positions x all {len(s['RA_shape_sample'])} shapes ({n} pairs, "
+                       f"{sec:.1f} s), numpy restatement of measure_w_lightcone.py:141-192",
+                seconds_full_estimate=sec * len(s["RA"]) / n_sample)
+
+
+def run_lightcone(a, w):
+    import torch
+    from measure_ia_amd.engine import get_engine
+    t0 = time.perf_counter()
+    s, lc = lightcone_inputs(w)
+    th, r_bins, sb = binning_for(w, None)
+    log(f"lightcone inputs built in {time.perf_counter() - t0:.1f} s")
+    eng = get_engine(0)
+    dev = torch.device("cuda", 0)
+    for k in ("ra", "dec", "chi", "cos_dec", "ra_s", "dec_s", "chi_s", "weight", "weight_shape", "c2", "s2"):
+        if getattr(lc, k) is not None:
+            setattr(lc, k, torch.from_numpy(np.ascontiguousarray(getattr(lc, k))).to(dev))
+    for k in ("patch", "patch_s"):
+        setattr(lc, k, torch.from_numpy(getattr(lc, k)).to(dev))
+    torch.cuda.synchronize()
+    for _ in range(a.warmup):
+        res = eng.count_lightcone(lc, th)
+    torch.cuda.synchronize()
+    kst = []
+    t1 = time.perf_counter()
+    for _ in range(a.steps):
+        res = eng.count_lightcone(lc, th)
+        kst.append(res.stats)
+    torch.cuda.synchronize()
+    ms_step = (time.perf_counter() - t1) / a.steps * 1e3
+    pairs = int(res.npairs[0].sum())
+    value = pairs / (ms_step / 1e3)
+    k_ms = float(np.mean([x["ms_pairs"] for x in kst]))
+    achieved = pairs * FLOP_PER_PAIR_LC / (k_ms / 1e3) / 1e12
+    log(f"lightcone: {ms_step:.2f} ms/step, pair kernel {k_ms:.2f} ms, {pairs} pairs")
+    cpu = None
+    if not a.no_cpu:
+        # about 10 s of one host core of an MI355X node: the loop's cost is one
+        # pass over all shapes per position
+        cpu = lightcone_cpu_baseline(s, w, r_bins, sb, n_sample=int(np.clip(1.2e10 / w["n_shape"] / 8, 8, 8000)))
+        log(f"cpu baseline: {cpu['value']:.3e} pairs/s on 1 core")
+    st = res.stats
+    line = {
+        "metric": "pairs/s (lightcone S+D count, 27 patches)", "value": value, "unit": "pairs/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{a.workload}: {w['desc']}", "mode": w["mode"], "patches": lc.n_patches,
+                   "positions": w["n_pos"], "shapes": w["n_shape"], "pairs": pairs},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": measured_traffic(a.workload, "k_lc_pairs"),
+                     "kernel": "k_lc_pairs",
+                     "kernel_ms": k_ms, "candidate_efficiency": pairs / max(1, st["candidates"]),
+                     "model": f"{FLOP_PER_PAIR_LC} FLOP per in-domain pair; FP64 vector peak"},
+        "cpu_baseline": cpu,
+        "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
+        "engine": dict(st),
+    }
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -165,6 +291,9 @@ def main():
     from measure_ia_amd.engine import SampleArrays, get_engine
 
     w = WORKLOADS[a.workload]
+    if w["kind"] == "lightcone":
+        run_lightcone(a, w)
+        return
     batch = w["kind"] == "batch"
     jk = w["jk"]
     t0 = time.perf_counter()

@@ -951,8 +951,11 @@ void count_lightcone(Ctx &ctx, const mia_lc_sample &s, const mia_bins &bins, con
     const double ra_span = std::fmax(ra_hi - ra_lo, 1e-9);
     const double dec_mid = 0.5 * (dec_lo + dec_hi);
     const double cmid = std::fmax(std::cos(dec_mid * kDeg), 0.05);
-    // cell: half the typical reach, but at least ~cell_shapes shapes per cell
-    double cell = 0.5 * theta;
+    // cell: the angular reach at the typical distance, but at least ~cell_shapes
+    // shapes per cell.  Finer cells cut candidates but multiply work items and
+    // ranges; on MI355X one typical reach per cell gave the shortest counts
+    // (pair kernel ~1.4 ps per candidate plus ~1.2 ns per item).
+    double cell = theta;
     const double want = tuning && tuning->cell_shapes > 0 ? tuning->cell_shapes : 16.0;
     const double area = dec_span * ra_span * cmid;
     const double cell_occ = std::sqrt(area * want / (double)ns_);
@@ -975,8 +978,18 @@ void count_lightcone(Ctx &ctx, const mia_lc_sample &s, const mia_bins &bins, con
     g.hr = ra_span / ncols;
     g.eps_deg = 1e-9 * (std::fabs(dec_lo) + std::fabs(dec_hi) + std::fabs(ra_lo) + std::fabs(ra_hi) + 1.0);
     g.P1 = P1;
-    // position slabs along chi: a block spans at most half the chi window by default
-    const int eighths = tuning && tuning->slabs_per_block > 0 ? tuning->slabs_per_block : 4;
+    // position slabs along chi, in eighths of the chi window: deeper slabs put
+    // more positions in a block (fewer work items) at the price of a wider
+    // window; measured best near 16 positions per (cell, slab) segment
+    int eighths = 4;
+    {
+        const double ncell = (double)std::max(1, std::min(nrows, 4096)) * std::max(1, std::min(ncols, 4096));
+        const double nsl4 = std::floor((pchi_hi - pchi_lo) / std::fmax((g.whi - g.wlo) * 0.5, 1e-9)) + 1;
+        const double per_seg = (double)np_ / (ncell * nsl4);
+        eighths = (int)std::lround(4.0 * std::sqrt(16.0 / std::fmax(per_seg, 1e-3)));
+        eighths = std::max(2, std::min(eighths, 8));
+    }
+    if (tuning && tuning->slabs_per_block > 0) eighths = tuning->slabs_per_block;
     g.dchi = std::fmax((g.whi - g.wlo) * eighths / 8.0, 1e-9);
     g.chi0 = pchi_lo;
     double nsl = std::floor((pchi_hi - pchi_lo) / g.dchi) + 1;

@@ -198,6 +198,9 @@ def lightcone_points(rng, n, ra=(10.0, 40.0), dec=(-10.0, 10.0), chi=(200.0, 140
         members = rng.integers(0, k, n_cl)
         x = centres[members] + rng.normal(0.0, sigma, (n_cl, 3))
         mr, md, mc = _cart_to_sky(x)
+        # keep RA continuous across the wedge (no jump from 0 to 360)
+        mid = 0.5 * (ra[0] + ra[1])
+        mr = (mr - mid + 180.0) % 360.0 - 180.0 + mid
         # keep members inside the wedge's chi range
         mc = np.clip(mc, chi[0], chi[1])
         r, d, c = np.concatenate([r, mr]), np.concatenate([d, md]), np.concatenate([c, mc])
