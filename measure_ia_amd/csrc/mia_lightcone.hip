@@ -1074,6 +1074,11 @@ void count_lightcone(Ctx &ctx, const mia_lc_sample &s, const mia_bins &bins, con
     int64_t total = 0;
     MIA_HIP(hipMemcpyAsync(&total, soff + nblocks, 8, hipMemcpyDeviceToHost, st));
     MIA_HIP(hipStreamSynchronize(st));
+    // positions at comoving distances near zero reach the whole sky; too many
+    // of them would need more (block, cell) pairs than one planning pass holds
+    if (total > (int64_t(1) << 31) - 2)
+        throw Error{MIA_E_UNSUPPORTED,
+                    "angular reach of the positions covers too many cells (comoving distances near zero?)"};
 
     // ---- plan: ranges (count, scan, emit)
     LcPlan L{};
