@@ -622,24 +622,34 @@ __device__ __forceinline__ void run_item(const PairArgs &P, const SampleView &S,
         int p0i = p0 + (v0 ? pi0 : 0);
         double pa0 = S.pa[p0i], pb0 = S.pb[p0i], pl0 = S.pl[p0i];
         double pw0 = WEIGHTED ? S.pw[p0i] : 1.0;
+        // two shapes per iteration, the record after them loaded ahead (scalar
+        // loads while the current pairs are evaluated)
         double qa = shp[0], qb = shp[1], ql = shp[2], qax = shp[3], qay = shp[4], qew = shp[5],
                qw = shp[6], q2ew = shp[7];
-        for (int j = 0; j < ns; ++j) {
-            // prefetch the next shape record (scalar loads) while this one is used
-            ShapePtr rn = shp + 8 * (j + 1 < ns ? j + 1 : j);
-            const double na = rn[0], nbv = rn[1], nl = rn[2], nax = rn[3], nay = rn[4],
-                         new_ = rn[5], nw = rn[6], n2ew = rn[7];
+        int j = 0;
+        for (; j + 1 < ns; j += 2) {
+            ShapePtr r1 = shp + 8 * (j + 1);
+            const double b0 = r1[0], b1 = r1[1], b2 = r1[2], b3 = r1[3], b4 = r1[4], b5 = r1[5],
+                         b6 = r1[6], b7 = r1[7];
             one_pair<MODE, SH, WEIGHTED, VAR>(P, S, v0, pa0, pb0, pl0, pw0, qa, qb, ql, qax, qay, qew, qw,
                                          q2ew, sha, shb, shl, hist, T, X, copy, nre, pd);
-            qa = na;
-            qb = nbv;
-            ql = nl;
-            qax = nax;
-            qay = nay;
-            qew = new_;
-            qw = nw;
-            q2ew = n2ew;
+            ShapePtr r2 = shp + 8 * (j + 2 < ns ? j + 2 : j + 1);
+            const double c0 = r2[0], c1 = r2[1], c2 = r2[2], c3 = r2[3], c4 = r2[4], c5 = r2[5],
+                         c6 = r2[6], c7 = r2[7];
+            one_pair<MODE, SH, WEIGHTED, VAR>(P, S, v0, pa0, pb0, pl0, pw0, b0, b1, b2, b3, b4, b5, b6,
+                                         b7, sha, shb, shl, hist, T, X, copy, nre, pd);
+            qa = c0;
+            qb = c1;
+            ql = c2;
+            qax = c3;
+            qay = c4;
+            qew = c5;
+            qw = c6;
+            q2ew = c7;
         }
+        if (j < ns)
+            one_pair<MODE, SH, WEIGHTED, VAR>(P, S, v0, pa0, pb0, pl0, pw0, qa, qb, ql, qax, qay, qew, qw,
+                                         q2ew, sha, shb, shl, hist, T, X, copy, nre, pd);
     }
     if (pd.b >= 0) hist_add<WEIGHTED, VAR>(P, hist, pd.b, pd.c, pd.sp, pd.sx, pd.dd, pd.vr);
 }
