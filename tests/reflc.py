@@ -51,16 +51,18 @@ def _project(xi, mode, second):
     return gp, gg
 
 
-def expected(mode, est, D, S, R, chi, r_bins, second, r_min, r_max, patches, rp_cut=None):
+def expected(mode, est, D, S, R, chi, r_bins, second, r_min, r_max, patches, rp_cut=None, h=1.0,
+             over_h=False):
     """D, S, R: dicts of RA, DEC, w (+ e1, e2 for S); chi: dict of comoving
-    distances per sample; patches: dict position/shape/randoms."""
+    distances per sample (already times h with over_h); patches: dict
+    position/shape/randoms."""
     def cnt(pos, shp, pkey, skey, with_shapes, mp=None, ms=None):
         mp = slice(None) if mp is None else mp
         ms = slice(None) if ms is None else ms
         return olc.count(mode, pos["RA"][mp], pos["DEC"][mp], chi[pkey][mp], pos["w"][mp], shp["RA"][ms],
                          shp["DEC"][ms], chi[skey][ms], shp["w"][ms], r_bins, second, r_min, r_max,
                          e1=shp["e1"][ms] if with_shapes else None, e2=shp["e2"][ms] if with_shapes else None,
-                         rp_cut=rp_cut, clamp=mode == "rppi")
+                         rp_cut=rp_cut, clamp=mode == "rppi", h=h, over_h=over_h)
 
     def all_counts(mD=None, mS=None, mR=None):
         a = cnt(D, S, "D", "S", True, mD, mS)

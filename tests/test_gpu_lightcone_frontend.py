@@ -2,7 +2,7 @@
 
 One small lightcone (1,200 positions, 600 shapes, 3,000 randoms, 4 RA-band
 jackknife patches) through measure_xi_w and measure_xi_multipoles with both
-estimators; the HDF5 datasets -- counts, estimators, projections, every
+estimators and every correlation type (and once with over_h); the HDF5 datasets -- counts, estimators, projections, every
 realisation's counts and the jackknife covariance -- are compared with
 tests/reflc.py.  Pair counts of unit-weight samples must be equal; every
 other dataset within 1e-6 of its largest value.
@@ -55,19 +55,20 @@ def _close(got, ref, what, exact=False):
     assert np.abs(got - ref).max() <= 1e-6 * max(m, 1e-300), what
 
 
+@pytest.mark.parametrize("corr", ["both", "gg", "g+"])
 @pytest.mark.parametrize("group", ["w", "multipoles"])
 @pytest.mark.parametrize("est", ["galaxies", "clusters"])
-def test_lightcone_frontend(tmp_path, group, est):
+def test_lightcone_frontend(tmp_path, group, est, corr):
     data, randoms, patches = _catalogue()
     fn = str(tmp_path / "lc.hdf5")
     lc = MeasureIALightcone(data, randoms, fn, separation_limits=[0.5, 15.0], num_bins_r=8, num_bins_pi=10,
                             pi_max=20.0)
     jk = {k: v.copy() for k, v in patches.items()}
     if group == "w":
-        lc.measure_xi_w(est, "gp", "both", jk_patches=jk)
+        lc.measure_xi_w(est, "gp", corr, jk_patches=jk)
         mode, second = "rppi", lc.pi_bins
     else:
-        lc.measure_xi_multipoles(est, "gp", "both", jk_patches=jk, over_h=False)
+        lc.measure_xi_multipoles(est, "gp", corr, jk_patches=jk, over_h=False)
         mode, second = "rmu", lc.mu_r_bins
     got = _read(fn)
     cosmo = default_cosmology()
@@ -82,27 +83,70 @@ def test_lightcone_frontend(tmp_path, group, est):
     g = group
     m = ref["main"]
     assert m["RR"].sum() > 1000 and np.abs(m["SpD"]).sum() > 0
-    _close(got[f"{g}/xi_g_plus/gp_SplusD"], m["SpD"], "S+D")
-    _close(got[f"{g}/xi_g_cross/gp_ScrossD"], m["SxD"], "SxD")
-    _close(got[f"{g}/xi_gg/gp_DD"], m["DD"], "DD")
-    _close(got[f"{g}/xi_g_plus/gp_randoms_SplusD"], m["SpR"], "S+R")
-    _close(got[f"{g}/xi_gg/gp_randoms_DD"], m["SR"], "SR")
-    _close(got[f"{g}/xi_gg/gp_RD"], m["RD"], "RD")
-    _close(got[f"{g}/xi_gg/gp_RR"], m["RR"], "RR", exact=True)
-    _close(got[f"{g}/xi_g_plus/gp"], ref["xi_g_plus"], "xi_g+")
-    _close(got[f"{g}/xi_gg/gp"], ref["xi_gg"], "xi_gg")
+    gp = corr in ("both", "g+")
+    gg = corr in ("both", "gg")
     proj = "w" if g == "w" else "multipoles"
-    _close(got[f"{proj}_g_plus/gp"], ref[f"{proj}_g_plus"], "projection g+")
-    _close(got[f"{proj}_gg/gp"], ref[f"{proj}_gg"], "projection gg")
+    # the datasets each correlation type writes (measure_IA.py:454-545)
+    if gp:
+        _close(got[f"{g}/xi_g_plus/gp_SplusD"], m["SpD"], "S+D")
+        _close(got[f"{g}/xi_g_cross/gp_ScrossD"], m["SxD"], "SxD")
+        _close(got[f"{g}/xi_g_plus/gp_randoms_SplusD"], m["SpR"], "S+R")
+        _close(got[f"{g}/xi_gg/gp_randoms_DD"], m["SR"], "SR (S+R call)")
+        _close(got[f"{g}/xi_g_plus/gp"], ref["xi_g_plus"], "xi_g+")
+        _close(got[f"{proj}_g_plus/gp"], ref[f"{proj}_g_plus"], "projection g+")
+    else:
+        assert f"{g}/xi_g_plus/gp_SplusD" not in got
+    _close(got[f"{g}/xi_gg/gp_DD"], m["DD"], "DD")
+    if gg:
+        _close(got[f"{g}/xi_gg/gp_RD"], m["RD"], "RD")
+        _close(got[f"{g}/xi_gg/gp"], ref["xi_gg"], "xi_gg")
+        _close(got[f"{proj}_gg/gp"], ref[f"{proj}_gg"], "projection gg")
+    if corr == "gg":
+        _close(got[f"{g}/xi_gg/gp_SR"], m["SR"], "SR")
+    if gg or est == "galaxies":
+        _close(got[f"{g}/xi_gg/gp_RR"], m["RR"], "RR", exact=True)
     for i, (c, _, _) in ref["reals"].items():
-        _close(got[f"{g}/xi_g_plus/gp_jk{P}/gp_{i}_SplusD"], c["SpD"], f"S+D real {i}")
         _close(got[f"{g}/xi_gg/gp_jk{P}/gp_{i}_DD"], c["DD"], f"DD real {i}")
-        _close(got[f"{g}/xi_gg/gp_jk{P}/gp_{i}_RR"], c["RR"], f"RR real {i}", exact=True)
-        _close(got[f"{g}/xi_gg/gp_jk{P}/gp_{i}_RD"], c["RD"], f"RD real {i}")
-        _close(got[f"{g}/xi_g_plus/gp_randoms_jk{P}/gp_randoms_{i}_SplusD"], c["SpR"], f"S+R real {i}")
-    for name in ("g_plus", "gg"):
-        _close(got[f"{proj}_{name}/gp_mean_{P}"], ref[f"mean_{name}"], f"mean {name}")
-        _close(got[f"{proj}_{name}/gp_jackknife_cov_{P}"], ref[f"cov_{name}"], f"cov {name}")
+        if gp:
+            _close(got[f"{g}/xi_g_plus/gp_jk{P}/gp_{i}_SplusD"], c["SpD"], f"S+D real {i}")
+            _close(got[f"{g}/xi_g_plus/gp_randoms_jk{P}/gp_randoms_{i}_SplusD"], c["SpR"], f"S+R real {i}")
+        if gg:
+            _close(got[f"{g}/xi_gg/gp_jk{P}/gp_{i}_RD"], c["RD"], f"RD real {i}")
+        if gg or est == "galaxies":
+            _close(got[f"{g}/xi_gg/gp_jk{P}/gp_{i}_RR"], c["RR"], f"RR real {i}", exact=True)
+    for name, on in (("g_plus", gp), ("gg", gg)):
+        if on:
+            _close(got[f"{proj}_{name}/gp_mean_{P}"], ref[f"mean_{name}"], f"mean {name}")
+            _close(got[f"{proj}_{name}/gp_jackknife_cov_{P}"], ref[f"cov_{name}"], f"cov {name}")
+        else:
+            assert f"{proj}_{name}/gp_jackknife_cov_{P}" not in got
+
+
+def test_lightcone_over_h(tmp_path):
+    """over_h: chi times h, and r_p from h*(dx, dy) (measure_w_lightcone.py:126-153)."""
+    data, randoms, patches = _catalogue()
+    fn = str(tmp_path / "lc.hdf5")
+    lc = MeasureIALightcone(data, randoms, fn, separation_limits=[0.5, 15.0], num_bins_r=8, num_bins_pi=10,
+                            pi_max=20.0)
+    lc.measure_xi_w("galaxies", "gp", "both", jk_patches={k: v.copy() for k, v in patches.items()},
+                    over_h=True)
+    got = _read(fn)
+    cosmo = default_cosmology()
+    h = cosmo["h"]
+    chi = {"D": comoving_distance(cosmo, 1 / (1 + data["Redshift"])) * h,
+           "S": comoving_distance(cosmo, 1 / (1 + data["Redshift_shape_sample"])) * h,
+           "R": comoving_distance(cosmo, 1 / (1 + randoms["Redshift"])) * h}
+    D = {"RA": data["RA"], "DEC": data["DEC"], "w": data["weight"]}
+    S = {"RA": data["RA_shape_sample"], "DEC": data["DEC_shape_sample"], "w": data["weight_shape_sample"],
+         "e1": data["e1"], "e2": data["e2"]}
+    R = {"RA": randoms["RA"], "DEC": randoms["DEC"], "w": np.ones(len(randoms["RA"]))}
+    ref = expected("rppi", "galaxies", D, S, R, chi, lc.r_bins, lc.pi_bins, 0.5, 15.0, patches, h=h,
+                   over_h=True)
+    m = ref["main"]
+    _close(got["w/xi_g_plus/gp_SplusD"], m["SpD"], "S+D")
+    _close(got["w/xi_gg/gp_RR"], m["RR"], "RR", exact=True)
+    _close(got["w_g_plus/gp"], ref["w_g_plus"], "w_g+")
+    _close(got["w_g_plus/gp_jackknife_cov_4"], ref["cov_g_plus"], "cov g+")
 
 
 def test_lightcone_pooled_layout(tmp_path):
