@@ -64,3 +64,28 @@ def test_lightcone_tuning_does_not_change_counts(engine):
         r = engine.count_lightcone(lc, th, tuning=tu)
         np.testing.assert_array_equal(r.npairs, base.npairs)
         assert np.abs(r.splus - base.splus).max() <= 1e-12 * np.abs(base.splus).max()
+
+
+def test_lightcone_fullsize_leave_one_patch_out(engine):
+    """4,000,000 randoms in a 60 x 30 degree wedge (the bench's RR count, about
+    1.6e9 pairs), where the oracle would take days: a fresh count without the
+    objects of patch i equals total - tally[i] (measure_jackknife.py:84-130
+    builds realisation i that way), exactly for the integer pair counts."""
+    from measure_ia_amd import synth
+    from measure_ia_amd.engine import LightconeArrays
+    rng = np.random.default_rng(13)
+    ra, dec, chi = synth.lightcone_points(rng, 4_000_000, ra=(0.0, 60.0), dec=(-15.0, 15.0),
+                                          chi=(300.0, 2000.0), clustered_frac=0.0)
+    patch = (np.clip((ra / 60.0 * 9).astype(int), 0, 8) * 3 + np.clip(((dec + 15.0) / 30.0 * 3).astype(int), 0, 2))
+    patch = patch.astype(np.int32)
+    _, _, th = bins(dict(mode="rppi", r_min=0.1, r_max=20.0, nr=10, n2=20, pi_max=60.0))
+    cosd = np.cos(dec / 180 * np.pi)
+    full = engine.count_lightcone(LightconeArrays(ra=ra, dec=dec, chi=chi, cos_dec=cosd, ra_s=ra, dec_s=dec,
+                                                  chi_s=chi, patch=patch, patch_s=patch, n_patches=27), th)
+    assert full.npairs[0].sum() > 1_000_000_000
+    for i in (0, 13, 26):
+        keep = patch != i
+        sub = engine.count_lightcone(LightconeArrays(ra=ra[keep], dec=dec[keep], chi=chi[keep], cos_dec=cosd[keep],
+                                                     ra_s=ra[keep], dec_s=dec[keep], chi_s=chi[keep]), th)
+        np.testing.assert_array_equal(sub.npairs[0], full.npairs[0] - full.npairs[1 + i])
+        np.testing.assert_array_equal(sub.dd[0], full.dd[0] - full.dd[1 + i])
