@@ -81,9 +81,14 @@ class FlatLCDM:
         return self._grid
 
     def comoving_radial_distance(self, a):
+        """Linear interpolation of the table (uniform in ln a; clamped at its ends)."""
         a = np.asarray(a, dtype=np.float64)
         x, chi = self._table()
-        return np.interp(np.log(a), x, chi)
+        n = len(x)
+        t = np.clip((np.log(a) - x[0]) / (x[1] - x[0]), 0.0, n - 1.0)
+        i = np.minimum(t.astype(np.int64), n - 2)
+        f = t - i
+        return chi[i] + f * (chi[i + 1] - chi[i])
 
 
 def comoving_distance(cosmology, a):
@@ -185,6 +190,7 @@ class MeasureIALightcone(SimInfo):
         self.num_samples = None
         self.device = device
         self._th = {}
+        self._memo = None   # per-measurement cache of converted data columns
 
     # shared post-processing of the box front-end (measure_IA_base.py:515-668)
     _store = MeasureIABox._store
@@ -254,7 +260,31 @@ class MeasureIALightcone(SimInfo):
         out["weight_shape_sample"] = np.asarray(self.data["weight_shape_sample"])[masks["weight_shape_sample"]]
         return out
 
-    def _arrays(self, sel, shapes, over_h, cosmology, print_num, rp_cut=None, patches=None):
+    def _column(self, tag, keys, masks, fn):
+        """fn of whole data columns (elementwise), then the selection of keys[0].
+
+        Within one measurement the columns of self.data are converted once
+        (comoving distances, cos dec, rotated ellipticities) and every count
+        selects from the result: fn(x)[mask] equals fn(x[mask]) elementwise.
+        Masks that differ between the keys are applied first instead."""
+        full = [np.asarray(self.data[k]) for k in keys]
+        m0 = None if masks is None else masks[keys[0]]
+        if masks is not None and any(not (masks[k] is m0 or np.array_equal(masks[k], m0)) for k in keys[1:]):
+            return fn(*[f[masks[k]] for f, k in zip(full, keys)])
+        memo = getattr(self, "_memo", None)
+        key = (tag,) + tuple(id(f) for f in full)
+        hit = memo.get(key) if memo is not None else None
+        if hit is not None and all(a is b for a, b in zip(hit[0], full)):
+            val = hit[1]
+        else:
+            val = fn(*full)
+            if memo is not None:
+                memo[key] = (full, val)
+        if m0 is None:
+            return val
+        return tuple(v[m0] for v in val) if isinstance(val, tuple) else val[m0]
+
+    def _arrays(self, sel, shapes, over_h, cosmology, print_num, rp_cut=None, patches=None, masks=None):
         """LightconeArrays of a selection (measure_w_lightcone.py:118-139)."""
         n_p, n_s = len(sel["RA"]), len(sel["RA_shape_sample"])
         if print_num:
@@ -264,20 +294,24 @@ class MeasureIALightcone(SimInfo):
             if print_num:
                 print("No cosmology given, using Omega_m=0.27, Omega_b=0.045, sigma8=0.8, h=0.7, n_s=1.")
         h = cosmology["h"]
-        chi = comoving_distance(cosmology, 1 / (1 + np.asarray(sel["Redshift"], dtype=np.float64)))
-        chi_s = comoving_distance(cosmology, 1 / (1 + np.asarray(sel["Redshift_shape_sample"],
-                                                                dtype=np.float64)))
-        if over_h:
-            chi = chi * h
-            chi_s = chi_s * h
+
+        def dist(z):
+            chi = comoving_distance(cosmology, 1 / (1 + np.asarray(z, dtype=np.float64)))
+            return chi * h if over_h else chi
+
+        ctag = ("chi", id(cosmology), bool(over_h))
+        chi = self._column(ctag, ["Redshift"], masks, dist)
+        chi_s = self._column(ctag, ["Redshift_shape_sample"], masks, dist)
         dec = np.asarray(sel["DEC"], dtype=np.float64)
+        cos_dec = self._column("cos", ["DEC"], masks,
+                               lambda d: np.cos(np.asarray(d, dtype=np.float64) / 180 * np.pi))
         c2 = s2 = None
         if shapes:
-            c2, s2 = rotated_ellipticity(np.asarray(sel["e1"], dtype=np.float64),
-                                         np.asarray(sel["e2"], dtype=np.float64))
+            c2, s2 = self._column("ell", ["e1", "e2"], masks, lambda a, b: rotated_ellipticity(
+                np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)))
         lc = LightconeArrays(
             ra=np.asarray(sel["RA"], dtype=np.float64), dec=dec, chi=chi,
-            cos_dec=np.cos(dec / 180 * np.pi),
+            cos_dec=cos_dec,
             ra_s=np.asarray(sel["RA_shape_sample"], dtype=np.float64),
             dec_s=np.asarray(sel["DEC_shape_sample"], dtype=np.float64), chi_s=chi_s,
             weight=np.asarray(sel["weight"], dtype=np.float64),
@@ -289,7 +323,7 @@ class MeasureIALightcone(SimInfo):
 
     def _count(self, mode, masks, shapes, over_h, cosmology, print_num, rp_cut=None, patches=None):
         sel = self._select(masks, shapes)
-        lc = self._arrays(sel, shapes, over_h, cosmology, print_num, rp_cut, patches)
+        lc = self._arrays(sel, shapes, over_h, cosmology, print_num, rp_cut, patches, masks)
         return self._engine().count_lightcone(lc, self._thresholds(mode))
 
     def _centres(self, mode):
@@ -401,12 +435,12 @@ class MeasureIALightcone(SimInfo):
         (measure_jackknife.py:84-110); it is the total minus tally[i]."""
         patches_pos = np.asarray(patches_pos)
         patches_shape = np.asarray(patches_shape)
-        min_patch, max_patch = int(min(patches_pos)), int(max(patches_pos))
+        min_patch, max_patch = int(np.min(patches_pos)), int(np.max(patches_pos))
         num_patches = max_patch - min_patch + 1
-        if min(patches_shape) != min_patch:
+        if np.min(patches_shape) != min_patch:
             print("Warning! Minimum patch number of shape sample is not equal to minimum patch number of "
                   "position sample.")
-        if max(patches_shape) != max_patch:
+        if np.max(patches_shape) != max_patch:
             print("Warning! Maximum patch number of shape sample is not equal to maximum patch number of "
                   "position sample.")
         mp = np.ones(len(patches_pos), dtype=bool)
@@ -415,9 +449,18 @@ class MeasureIALightcone(SimInfo):
             # the reference's (patches != i) * masks["Redshift"] (raises like it on a length mismatch)
             mp = np.asarray(mp * masks["Redshift"], dtype=bool)
             ms = np.asarray(ms * masks["Redshift_shape_sample"], dtype=bool)
+        # sum(mask) of every realisation (measure_jackknife.py:89-90): the
+        # selected objects minus those of patch i, counted in one pass
+        def per_patch(labels, sel):
+            lab = labels[sel]
+            inside = (lab >= min_patch) & (lab <= max_patch)
+            return lab.size, np.bincount((lab[inside] - min_patch).astype(np.int64), minlength=num_patches)
+
+        n_s, c_s = per_patch(patches_shape, ms)
+        n_p, c_p = per_patch(patches_pos, mp)
         for i in range(min_patch, max_patch + 1):
-            self.num_samples[f"{i}"][num_sample_names[0]] = int(np.sum((patches_shape != i) & ms))
-            self.num_samples[f"{i}"][num_sample_names[1]] = int(np.sum((patches_pos != i) & mp))
+            self.num_samples[f"{i}"][num_sample_names[0]] = int(n_s - c_s[i - min_patch])
+            self.num_samples[f"{i}"][num_sample_names[1]] = int(n_p - c_p[i - min_patch])
         keys = {"Redshift": mp, "Redshift_shape_sample": ms, "RA": mp, "RA_shape_sample": ms, "DEC": mp,
                 "DEC_shape_sample": ms, "weight": mp, "weight_shape_sample": ms}
         if shapes:
@@ -695,6 +738,7 @@ class MeasureIALightcone(SimInfo):
     def _run(self, group, IA_estimator, dataset_name, corr_type, jk_patches, num_jk, errors, masks,
              masks_randoms, cosmology, over_h, rp_cut):
         mode = "rppi" if group == "w" else "rmu"
+        self._memo = {}
         data, jk_patches, num_samples = self._setup(IA_estimator, masks, masks_randoms, jk_patches, num_jk,
                                                     errors)
         if mode == "rppi":
@@ -740,11 +784,12 @@ class MeasureIALightcone(SimInfo):
                              over_h, rp_cut)
         finally:
             self.data = data
+            self._memo = None
 
     def _errors(self, group, IA_estimator, dataset_name, corr_type, jk_patches, masks, cosmology, over_h,
                 rp_cut):
         self.num_samples = {}
-        mn, mx = int(min(jk_patches["shape"])), int(max(jk_patches["shape"]))
+        mn, mx = int(np.min(jk_patches["shape"])), int(np.max(jk_patches["shape"]))
         for n in range(mn, mx + 1):
             self.num_samples[f"{n}"] = {}
         real = self._measure_jackknife_realisations_lightcone_multiprocessing
@@ -779,8 +824,9 @@ class MeasureIALightcone(SimInfo):
             real(patches_pos=jk_patches["randoms_position"], patches_shape=jk_patches["randoms_shape"],
                  corr_type=["gg", group], dataset_name=dataset_name, count_pairs=True, data_suffix="_RR",
                  num_sample_names=["R_S", "R_D"], **kw)
-        self._measure_jackknife_covariance_lightcone(IA_estimator=IA_estimator, max_patch=max(jk_patches["shape"]),
-                                                     min_patch=min(jk_patches["shape"]),
+        self._measure_jackknife_covariance_lightcone(IA_estimator=IA_estimator,
+                                                     max_patch=int(np.max(jk_patches["shape"])),
+                                                     min_patch=int(np.min(jk_patches["shape"])),
                                                      corr_type=[corr_type, group], dataset_name=dataset_name,
                                                      randoms_suf="_randoms")
 
