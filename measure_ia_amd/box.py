@@ -388,6 +388,49 @@ class MeasureIABox(SimInfo):
         if return_output:
             return covs, stds
 
+    def measure_gg_quadrupole(self, dataset_name, num_jk=0, return_output=False):
+        """l = 2 multipole of xi_gg(r, mu_r), read from the stored grids.
+
+        An extension: the reference writes only l = 0 for gg
+        (measure_IA_base.py:577-668).  The same weights are applied with
+        P_2(mu) = (3 mu^2 - 1)/2 at the stored mu centres:
+            xi_gg,2(r) = (2l+1)/2 sum_mu P_2(mu_c) xi_gg(r, mu) dmu.
+        Writes multipoles_gg_l2/{dataset_name} (and _r); with num_jk > 0 also
+        every realisation under multipoles_gg_l2/{dataset_name}_jk{num_jk} and
+        their mean, std and covariance with the (n-1)/n rule, as for l = 0.
+        Call after measure_xi_multipoles wrote the grids.
+        """
+        st, own = self._store(None)
+        try:
+            def one(name, jk_group):
+                base = f"{self.snap_group}/multipoles/xi_gg/{jk_group}"
+                xi = st.read(f"{base}/{name}")
+                mu = st.read(f"{base}/{name}_mu_r")
+                return estimators.multipole(xi, mu, self.mu_r_bins, 2, 0)
+
+            sep = estimators.bin_centres(self.r_bins)
+            q = one(dataset_name, "")
+            if return_output:
+                return np.array([sep, q]).transpose()
+            g = f"{self.snap_group}/multipoles_gg_l2/"
+            st.write(g, dataset_name + "_r", sep)
+            st.write(g, dataset_name, q)
+            if num_jk > 0:
+                jk = f"{dataset_name}_jk{num_jk}"
+                reals = []
+                for i in range(num_jk):
+                    qi = one(f"{dataset_name}_{i}", jk)
+                    st.write(g + jk, f"{dataset_name}_{i}_r", sep)
+                    st.write(g + jk, f"{dataset_name}_{i}", qi)
+                    reals.append(qi)
+                mean, std, cov = estimators.jackknife_combine(reals, self.num_bins_r)
+                st.write(g, dataset_name + "_mean_" + str(num_jk), mean)
+                st.write(g, dataset_name + "_jackknife_" + str(num_jk), std)
+                st.write(g, dataset_name + "_jackknife_cov_" + str(num_jk), cov)
+        finally:
+            if own:
+                st.close()
+
     # -- jackknife covariance across datasets (measure_jackknife.py:485-648) -----------
     def measure_covariance_multiple_datasets(self, corr_type, dataset_names, num_box=27,
                                              return_output=False):
