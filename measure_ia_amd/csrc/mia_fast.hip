@@ -545,7 +545,7 @@ __device__ __forceinline__ void one_pair(const PairArgs &P, const SampleView &S,
         // the guard band of an integer (an edge, or a slightly negative
         // guess at the lower domain edge) takes the exact path
         int ir = (int)gr, i2 = (int)g2;
-        bool near = (fabsf(gr - rintf(gr)) < P.dr) | (fabsf(g2 - rintf(g2)) < P.d2);
+        bool near = ((int)(fabsf(gr - rintf(gr)) < P.dr) | (int)(fabsf(g2 - rintf(g2)) < P.d2)) != 0;
         if (near) {
             ir = count_edges(T, n_r, rv);
             if (MODE == MIA_RPPI) {

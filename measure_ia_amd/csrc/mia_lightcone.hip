@@ -639,7 +639,7 @@ __device__ __forceinline__ void lc_pair(const LcPairArgs &P, bool valid, double 
     float g2 = MODE == MIA_RPPI ? __fmaf_rn(pif, P.xs, P.xc)
                                 : __fmaf_rn(pif, __builtin_amdgcn_rsqf(rv32), 1.0f) * P.xs;
     int ir = (int)gr, i2 = (int)g2;
-    bool near = (fabsf(gr - rintf(gr)) < P.dr) | (fabsf(g2 - rintf(g2)) < P.d2);
+    bool near = ((int)(fabsf(gr - rintf(gr)) < P.dr) | (int)(fabsf(g2 - rintf(g2)) < P.d2)) != 0;
     if (MODE == MIA_RMU) near |= s * (P.h * P.h) <= P.cut_hi;
     if (near) {
         // the reference's operation order (measure_w_lightcone.py:143-150)
