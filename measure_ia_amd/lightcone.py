@@ -17,14 +17,12 @@ provides them, and any object with ``comoving_radial_distance(a)`` and
 from __future__ import annotations
 
 import math
-from typing import Optional
 
 import numpy as np
 
 from . import binning, estimators
 from .box import MeasureIABox
 from .engine import LightconeArrays
-from .hdf5io import H5Store
 from .siminfo import SimInfo
 
 _C_KMS = 299792.458

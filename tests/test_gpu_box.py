@@ -11,7 +11,7 @@ import pytest
 
 from measure_ia_amd import MeasureIABox, ReadData
 from measure_ia_amd.hdf5io import H5Store
-from measure_ia_amd.synth import clustered, ellipticity_noise, shapes_from_components, wrap_box
+from measure_ia_amd.synth import ellipticity_noise, shapes_from_components, wrap_box
 from refbox import expected_output
 
 pytestmark = pytest.mark.gpu
