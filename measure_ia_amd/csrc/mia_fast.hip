@@ -360,6 +360,7 @@ struct PairArgs {
     float lgs, lgc;              // r bin guess: log2(v) * lgs + lgc
     float xs, xc;                // pi: pi * xs + xc; mu: (mu + 1) * xs
     float dr, d2;                // guard bands: exact path when |guess - nearest integer| < d
+    int scan;                    // 1: the exact path scans every threshold (guard band of a whole bin)
     const SampleDev *samples;
     int hist_stride;             // bytes per wave histogram
     int copies_log2;             // each bin is kept in 2^copies_log2 interleaved copies
@@ -445,6 +446,17 @@ __device__ __forceinline__ int count_edges(const double *E, int n, double v) {
     int k = 0;
     for (int i = 1; i < n; ++i) k += (E[i] <= v);
     return k;
+}
+
+// The same count from a guess g of v's position in bin units that is off by
+// less than half a bin: with e the integer nearest g, every edge below E[e]
+// lies below v and every edge above it above v, so one comparison decides.
+// v lies in [E[0], E[n]) (the domain test), so e = 0 and e = n need none.
+__device__ __forceinline__ int edge_near(const double *E, int n, double v, float g) {
+    const int e = (int)rintf(g);
+    if (e <= 0) return 0;
+    if (e >= n) return n - 1;
+    return e - 1 + (E[e] <= v ? 1 : 0);
 }
 
 #define MIA_CONST __attribute__((address_space(4)))
@@ -547,12 +559,22 @@ __device__ __forceinline__ void one_pair(const PairArgs &P, const SampleView &S,
         int ir = (int)gr, i2 = (int)g2;
         bool near = ((int)(fabsf(gr - rintf(gr)) < P.dr) | (int)(fabsf(g2 - rintf(g2)) < P.d2)) != 0;
         if (near) {
-            ir = count_edges(T, n_r, rv);
-            if (MODE == MIA_RPPI) {
-                i2 = count_edges(X, n_2, dl);
+            if (P.scan) {
+                ir = count_edges(T, n_r, rv);
+                if (MODE == MIA_RPPI) {
+                    i2 = count_edges(X, n_2, dl);
+                } else {
+                    double mu64 = dl / sqrt(rv);
+                    i2 = mu64 < X[0] ? n_2 - 1 : count_edges(X, n_2, mu64);
+                }
             } else {
-                double mu64 = dl / sqrt(rv);
-                i2 = mu64 < X[0] ? n_2 - 1 : count_edges(X, n_2, mu64);
+                ir = edge_near(T, n_r, rv, gr);
+                if (MODE == MIA_RPPI) {
+                    i2 = edge_near(X, n_2, dl, g2);
+                } else {
+                    double mu64 = dl / sqrt(rv);
+                    i2 = mu64 < X[0] ? n_2 - 1 : edge_near(X, n_2, mu64, g2);
+                }
             }
             ++nre;
         }
@@ -1010,6 +1032,7 @@ void count_fast(Ctx &ctx, std::vector<FastSample> &samples, const mia_bins &bins
     bool exact_only = dr > 0.05 || d2 > 0.05;
     A.dr = exact_only ? 1.0f : (float)dr;
     A.d2 = exact_only ? 1.0f : (float)d2;
+    A.scan = exact_only ? 1 : 0;
     A.samples = sdd;
     // histogram copies (default 1: the pending-slot registers already absorb
     // most collisions, and a single copy leaves LDS for 8 waves per SIMD);

@@ -546,6 +546,7 @@ struct LcPairArgs {
     double h;                    // projected-separation scale (h with over_h)
     double cut, cut_lo, cut_hi;  // (r, mu): r_p^2 threshold of rp_cut and its guard band
     float lgs, lgc, xs, xc, dr, d2;
+    int scan;                    // 1: the exact path scans every threshold (guard band of a whole bin)
     int hist_stride, off_dd, off_sp, off_sx;
     unsigned long long *npairs;  // (1+P)*B
     double *dd, *sp, *sx;        // (1+P)*B (sp, sx NULL for pair counts)
@@ -612,6 +613,18 @@ __device__ __forceinline__ int lc_count_edges(const double *E, int n, double v) 
     return k;
 }
 
+// The same count from a guess g of v's position in bin units that is off by
+// less than half a bin (the FP32 guess of the approximate separation: the
+// approximation differs from the exact value by a few ulps): with e the
+// integer nearest g, one comparison with E[e] decides.  v lies in
+// [E[0], E[n]), so e = 0 and e = n need none.
+__device__ __forceinline__ int lc_edge_near(const double *E, int n, double v, float g) {
+    const int e = (int)rintf(g);
+    if (e <= 0) return 0;
+    if (e >= n) return n - 1;
+    return e - 1 + (E[e] <= v ? 1 : 0);
+}
+
 template <int MODE, bool SHAPES>
 __device__ __forceinline__ void lc_pair(const LcPairArgs &P, bool valid, double qra, double qdec,
                                         double qchi, double qc2, double qs2, double qw, double pra,
@@ -651,15 +664,15 @@ __device__ __forceinline__ void lc_pair(const LcPairArgs &P, bool valid, double 
         const double es = hx * hx + hy * hy;
         if (MODE == MIA_RPPI) {
             if (!(es >= T[0] && es < T[n_r])) return;
-            ir = lc_count_edges(T, n_r, es);
-            i2 = lc_count_edges(X, n_2, pi);
+            ir = P.scan ? lc_count_edges(T, n_r, es) : lc_edge_near(T, n_r, es, gr);
+            i2 = P.scan ? lc_count_edges(X, n_2, pi) : lc_edge_near(X, n_2, pi, g2);
         } else {
             // (r, mu): r from the unscaled separation (measure_m_lightcone.py:136-141)
             const double r2 = (edx * edx + edy * edy) + pi * pi;
             if (!(es > P.cut && r2 >= T[0] && r2 < T[n_r])) return;
-            ir = lc_count_edges(T, n_r, r2);
+            ir = P.scan ? lc_count_edges(T, n_r, r2) : lc_edge_near(T, n_r, r2, gr);
             const double mu = pi / sqrt(r2);
-            i2 = mu < X[0] ? n_2 - 1 : lc_count_edges(X, n_2, mu);
+            i2 = mu < X[0] ? n_2 - 1 : (P.scan ? lc_count_edges(X, n_2, mu) : lc_edge_near(X, n_2, mu, g2));
         }
         ++nre;
     }
@@ -1229,6 +1242,7 @@ void count_lightcone(Ctx &ctx, const mia_lc_sample &s, const mia_bins &bins, con
     bool exact_only = dr > 0.05 || d2 > 0.05;
     A.dr = exact_only ? 1.0f : (float)dr;
     A.d2 = exact_only ? 1.0f : (float)d2;
+    A.scan = exact_only ? 1 : 0;
     int off = ((4 * B) + 15) & ~15;
     A.off_dd = off;
     off += 8 * B;

@@ -109,6 +109,30 @@ def jackknife_region_indices(positions, shapes, boxsize, L_subboxes):
     return np.array(jk_pos, dtype=int), np.array(jk_shape, dtype=int)
 
 
+def jackknife_region_indices_by_axis(positions, shapes, boxsize, L_subboxes):
+    """The same region ids with the triple loop factored per axis.
+
+    The reference's box test (measure_IA_base.py:404-452) is a product of one
+    strict interval test per axis, with the same bounds i*L_sub and
+    (i+1)*L_sub for every box, so an object's box index on each axis can be
+    found on its own: 3*L passes over the arrays instead of L^3.  An object
+    that passes no interval on some axis keeps region 0, as in the loop.
+    """
+    L_0p5 = boxsize / 2.0
+    L_sub = L_0p5 * 2.0 / L_subboxes
+
+    def ids(x):
+        idx = np.full(x.shape, -1, dtype=np.int64)
+        for i in np.arange(0, L_subboxes):
+            m = (x > i * L_sub) * (x < (i + 1) * L_sub)
+            idx[m] = i
+        ok = np.all(idx >= 0, axis=1)
+        reg = idx[:, 0] * L_subboxes * L_subboxes + idx[:, 1] * L_subboxes + idx[:, 2]
+        return np.where(ok, reg, 0).astype(int)
+
+    return ids(np.asarray(positions)), ids(np.asarray(shapes))
+
+
 def responsivity_jk(weight_shape, e, jk_shape, num_box):
     """R_jk[i] over shapes outside region i (measure_w_box_jk.py:213-216)."""
     R_jk = np.zeros(num_box)

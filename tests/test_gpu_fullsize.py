@@ -14,6 +14,11 @@
       LOS moved with them gives the same counts;
     - batched launch: 12 uneven weighted samples counted in one launch equal
       12 single counts.
+* The large box (20,000,000 clustered positions, L = 500, 15 log r_p bins
+  over 0.1-100, 40 pi bins over |pi| < 100, 125 regions): 160 seeded shapes
+  (16 chunks of 10) against all 20M positions, about 1.6e8 pairs, against the
+  oracle's KDTree counter -- the kernel at the large box's own density,
+  binning and region count, checked pair for pair.
 """
 import numpy as np
 import pytest
@@ -139,3 +144,30 @@ def test_batch12_equals_single_counts(engine):
         np.testing.assert_array_equal(b.npairs, one.npairs)
         assert np.abs(b.dd - one.dd).max() <= 1e-12 * np.abs(one.dd).max()
         assert np.abs(b.splus - one.splus).max() <= 1e-12 * np.abs(one.splus).max()
+
+
+def test_large20M_shape_chunks_against_tree_oracle(engine):
+    from oracle import prep, tree
+    cat = synth.config("large20M")
+    L = cat.boxsize
+    r_bins = np.logspace(np.log10(0.1), np.log10(100.0), 16)
+    sb = np.linspace(-100.0, 100.0, 41)
+    th = binning.rppi_thresholds(r_bins, sb, 0.1, 100.0)
+    pr = prep.prepare(cat.data)
+    jp, js = prep.jackknife_region_indices_by_axis(pr["positions"], pr["shapes"], L, 5)
+    chunk = 10
+    n_chunks = (len(pr["shapes"]) + chunk - 1) // chunk
+    pick = sorted(np.random.default_rng(5).choice(n_chunks, 16, replace=False).tolist())
+    sel = np.concatenate([np.arange(c * chunk, min((c + 1) * chunk, len(pr["shapes"]))) for c in pick])
+    ref = tree.count_pairs(pr, r_bins, sb, 0.1, 100.0, L, jk_pos=jp, jk_shape=js, num_box=125,
+                           chunk_size=chunk, num_nodes=min(16, tree.default_workers()), chunk_subset=pick,
+                           abs_sums=True)
+    print("pairs", ref["npairs"], "oracle s", round(ref["seconds"], 1), flush=True)
+    p = hostprep.prepare(cat.data)
+    res = engine.count(_sample(p, L, 5, None, sel), th)
+    assert res.npairs[0].sum() == ref["npairs"] > 100_000_000
+    np.testing.assert_array_equal(res.dd[0], ref["DD"])
+    np.testing.assert_array_equal(res.dd[1:], ref["DD_jk"])
+    assert _rel(res.splus[0] / (2 * p.R), ref["SplusD"], ref["SplusD_abs"]).max() <= 1e-6
+    assert _rel(res.scross / (2 * p.R), ref["ScrossD"], ref["ScrossD_abs"]).max() <= 1e-6
+    assert _rel(res.splus[1:], ref["SplusD_jk"], ref["SplusD_jk_abs"]).max() <= 1e-6

@@ -3,7 +3,9 @@
 3,600 (r_p, pi) bins no longer fit 4 wave histograms per workgroup, so the
 batched kernel runs 2 waves per workgroup; 10,000 bins do not fit even one,
 so mia_count counts them with the plain kernel (reported in stats.method) and
-the batched entry point refuses them.  Both agree with the oracle.
+the batched entry point refuses them.  Both agree with the oracle.  A thin
+shell (r in [10, 10.01) over 10 log bins) is binned by scanning every FP64
+threshold, since the FP32 guess is too coarse there.
 """
 import numpy as np
 import pytest
@@ -55,3 +57,33 @@ def test_large_binning(engine, nr, npi, method_used):
         with pytest.raises(MiaError) as e:
             engine.count_batch([s], th)
         assert e.value.code == -5
+
+
+@pytest.mark.parametrize("mode", ["rppi", "rmu"])
+def test_thin_shell_scans_every_threshold(engine, mode):
+    """r in [10, 10.01) over 10 log bins: the FP32 bin guess is no longer
+    within 5% of a bin, so every in-domain pair takes the exact FP64 path and
+    counts every threshold (stats.rechecks == pairs); counts equal the oracle."""
+    rng = np.random.default_rng(77)
+    pos = rng.uniform(0, L, (12000, 3))
+    shp = rng.uniform(0, L, (3000, 3))
+    th_ax = rng.uniform(0, np.pi, 3000)
+    d = {"Position": pos, "Position_shape_sample": shp, "LOS": 2, "q": rng.uniform(0.4, 1.0, 3000),
+         "Axis_Direction": np.stack([np.cos(th_ax), np.sin(th_ax)], 1)}
+    r_bins, pi_bins, mu_bins = prep.bin_edges(10.0, 10.01, 10, 8, 12.0, L)
+    second = pi_bins if mode == "rppi" else mu_bins
+    th = (binning.rppi_thresholds if mode == "rppi" else binning.rmu_thresholds)(r_bins, second, 10.0, 10.01)
+    p = hostprep.prepare(d)
+    s = SampleArrays(p.positions, p.shapes, p.axis, p.e, None, None, L, 2, 2, L / 2.0)
+    pr = prep.prepare(d)
+    jp, js = prep.jackknife_region_indices(pr["positions"], pr["shapes"], L, 2)
+    ref = brute.count_pairs(pr, r_bins, second, 10.0, 10.01, L, mode=mode, jk_pos=jp, jk_shape=js, num_box=8)
+    res = engine.count(s, th, method="fast")
+    assert res.stats["method"] == 1
+    assert ref["NP"].sum() > 2000
+    np.testing.assert_array_equal(res.npairs[0], ref["NP"])
+    np.testing.assert_array_equal(res.npairs[1:], ref["NP_jk"])
+    assert res.stats["rechecks"] == ref["NP"].sum()
+    err = np.abs(res.splus[0] / (2 * p.R) - ref["SplusD"]) / np.maximum(np.abs(ref["SplusD"]),
+                                                                        ref["SplusD_abs"] + 1e-300)
+    assert err.max() <= 1e-6

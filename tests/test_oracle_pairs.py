@@ -197,3 +197,26 @@ def test_open_box_scalar_loop_and_subset_of_periodic(mode):
     per = brute.count_pairs(pr, r_bins, sb, 0.2, 4.0, L, mode=mode, periodic=True)
     assert np.all(op["NP"] <= per["NP"])
     assert op["NP"].sum() < per["NP"].sum()
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5])
+def test_region_ids_by_axis_equal_the_triple_loop(n):
+    """The per-axis form of the region test gives the literal loop's ids,
+    also for objects on a face, at 0 and at L, and one ulp either side of a face."""
+    rng = np.random.default_rng(40 + n)
+    L = 205.0
+    x = rng.uniform(0, L, (6000, 3))
+    face = np.arange(n + 1) * (L / 2.0 * 2.0 / n)
+    k = rng.integers(0, n + 1, (1500, 3))
+    x[:1500] = face[k]
+    x[1500:2000] = np.nextafter(face[k[:500]], 0.0)
+    x[2000:2500] = np.nextafter(face[k[500:1000]], L + 1)
+    x[2500:2600] = 0.0
+    x[2600:2700] = L
+    x = np.clip(x, 0.0, L)
+    rng.shuffle(x, axis=1)
+    a_p, a_s = prep.jackknife_region_indices(x, x[::7], L, n)
+    b_p, b_s = prep.jackknife_region_indices_by_axis(x, x[::7], L, n)
+    np.testing.assert_array_equal(b_p, a_p)
+    np.testing.assert_array_equal(b_s, a_s)
+    assert (a_p > 0).sum() > 1000 or n == 1
