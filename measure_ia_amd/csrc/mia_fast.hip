@@ -517,7 +517,7 @@ __device__ __forceinline__ void one_pair(const PairArgs &P, const SampleView &S,
                                          double qb, double ql, double qax, double qay, double qew,
                                          double qw, double q2ew, double sha, double shb, double shl,
                                          char *hist, const double *T, const double *X, int copy,
-                                         uint32_t &nre, Pend &pd) {
+                                         uint32_t &nre, Pend &pd, float vlgc, float vxc) {
     const int n_r = P.n_r, n_2 = P.n_2;
     // the reference's own FP64 separation (measure_w_box_jk.py:159-162)
     double da = qa - pa, db = qb - pb, dl = ql - pl;
@@ -547,10 +547,10 @@ __device__ __forceinline__ void one_pair(const PairArgs &P, const SampleView &S,
         // included, so no clamping is needed)
         float rv32 = (float)rv;
         float dlf = (float)dl;
-        float gr = __fmaf_rn(__log2f(rv32), P.lgs, P.lgc);
+        float gr = __fmaf_rn(__log2f(rv32), P.lgs, vlgc);
         float g2;
         if (MODE == MIA_RPPI)
-            g2 = __fmaf_rn(dlf, P.xs, P.xc);
+            g2 = __fmaf_rn(dlf, P.xs, vxc);
         else
             g2 = __fmaf_rn(dlf, __builtin_amdgcn_rsqf(rv32), 1.0f) * P.xs;
         // truncation is floor for the non-negative guesses; a guess within
@@ -606,20 +606,18 @@ __device__ __forceinline__ void one_pair(const PairArgs &P, const SampleView &S,
         int b = ((__mul24(ir, n_2) + i2) << P.copies_log2) | copy;
         double dd = WEIGHTED ? pw * qw : 0.0;
         double vr = VAR ? ep * ep : 0.0;
-        if (b != pd.b) {
-            if (pd.b >= 0) hist_add<WEIGHTED, VAR>(P, hist, pd.b, pd.c, pd.sp, pd.sx, pd.dd, pd.vr);
-            pd.b = b;
-            pd.c = 0;
-            pd.sp = 0.0;
-            pd.sx = 0.0;
-            pd.dd = 0.0;
-            pd.vr = 0.0;
-        }
-        pd.c += 1;
-        pd.sp += ep;
-        pd.sx += ex;
-        if (WEIGHTED) pd.dd += dd;
-        if (VAR) pd.vr += vr;
+        // a new slot flushes the pending run; the run then restarts from this
+        // pair: keep = 0 drops the old sums (fma(0, sum, x) = x, fma(1, sum, x)
+        // = sum + x, both exactly as the plain assignment and addition)
+        const bool nw = b != pd.b;
+        if (nw & (pd.b >= 0)) hist_add<WEIGHTED, VAR>(P, hist, pd.b, pd.c, pd.sp, pd.sx, pd.dd, pd.vr);
+        const double keep = nw ? 0.0 : 1.0;
+        pd.b = b;
+        pd.c = (nw ? 0u : pd.c) + 1u;
+        pd.sp = fma(keep, pd.sp, ep);
+        pd.sx = fma(keep, pd.sx, ex);
+        if (WEIGHTED) pd.dd = fma(keep, pd.dd, dd);
+        if (VAR) pd.vr = fma(keep, pd.vr, vr);
     }
 }
 
@@ -633,6 +631,12 @@ __device__ __forceinline__ void run_item(const PairArgs &P, const SampleView &S,
     const double shb = SH == SH_CONST ? shift_val((shifts >> 2) & 3, L) : 0.0;
     const double shl = SH == SH_CONST ? shift_val((shifts >> 4) & 3, L) : 0.0;
     const int copy = lane & ((1 << P.copies_log2) - 1);
+    // the additive constants of the FP32 bin guesses, held in VGPRs for the
+    // whole item (a VOP3 FMA reads one scalar operand; the other would be
+    // copied into a VGPR on every pair)
+    float vlgc, vxc;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(vlgc) : "s"(P.lgc));
+    asm volatile("v_mov_b32 %0, %1" : "=v"(vxc) : "s"(P.xc));
     // Strided lane -> position map: lane l of batch b takes position b + l*nb,
     // so one batch spans the whole line-of-sight range of the item and its
     // pairs spread over more (r, pi) bins (fewer same-address LDS atomics).
@@ -654,12 +658,12 @@ __device__ __forceinline__ void run_item(const PairArgs &P, const SampleView &S,
             const double b0 = r1[0], b1 = r1[1], b2 = r1[2], b3 = r1[3], b4 = r1[4], b5 = r1[5],
                          b6 = r1[6], b7 = r1[7];
             one_pair<MODE, SH, WEIGHTED, VAR>(P, S, v0, pa0, pb0, pl0, pw0, qa, qb, ql, qax, qay, qew, qw,
-                                         q2ew, sha, shb, shl, hist, T, X, copy, nre, pd);
+                                         q2ew, sha, shb, shl, hist, T, X, copy, nre, pd, vlgc, vxc);
             ShapePtr r2 = shp + 8 * (j + 2 < ns ? j + 2 : j + 1);
             const double c0 = r2[0], c1 = r2[1], c2 = r2[2], c3 = r2[3], c4 = r2[4], c5 = r2[5],
                          c6 = r2[6], c7 = r2[7];
             one_pair<MODE, SH, WEIGHTED, VAR>(P, S, v0, pa0, pb0, pl0, pw0, b0, b1, b2, b3, b4, b5, b6,
-                                         b7, sha, shb, shl, hist, T, X, copy, nre, pd);
+                                         b7, sha, shb, shl, hist, T, X, copy, nre, pd, vlgc, vxc);
             qa = c0;
             qb = c1;
             ql = c2;
@@ -671,7 +675,7 @@ __device__ __forceinline__ void run_item(const PairArgs &P, const SampleView &S,
         }
         if (j < ns)
             one_pair<MODE, SH, WEIGHTED, VAR>(P, S, v0, pa0, pb0, pl0, pw0, qa, qb, ql, qax, qay, qew, qw,
-                                         q2ew, sha, shb, shl, hist, T, X, copy, nre, pd);
+                                         q2ew, sha, shb, shl, hist, T, X, copy, nre, pd, vlgc, vxc);
     }
     if (pd.b >= 0) hist_add<WEIGHTED, VAR>(P, hist, pd.b, pd.c, pd.sp, pd.sx, pd.dd, pd.vr);
 }

@@ -630,7 +630,7 @@ __device__ __forceinline__ void lc_pair(const LcPairArgs &P, bool valid, double 
                                         double qchi, double qc2, double qs2, double qw, double pra,
                                         double pdec, double pchi, double pax, double pay, double pw,
                                         double pcos, char *hist, const double *T, const double *X,
-                                        uint32_t &nre, LcPend &pd) {
+                                        uint32_t &nre, LcPend &pd, float vlgc, float vxc) {
     const int n_r = P.n_r, n_2 = P.n_2;
     const double dra = qra - pra, ddec = qdec - pdec;
     const double pi = qchi - pchi;  // the reference's LOS, exactly
@@ -648,8 +648,8 @@ __device__ __forceinline__ void lc_pair(const LcPairArgs &P, bool valid, double 
     }
     if (!in) return;
     float rv32 = (float)rv, pif = (float)pi;
-    float gr = __fmaf_rn(__log2f(rv32), P.lgs, P.lgc);
-    float g2 = MODE == MIA_RPPI ? __fmaf_rn(pif, P.xs, P.xc)
+    float gr = __fmaf_rn(__log2f(rv32), P.lgs, vlgc);
+    float g2 = MODE == MIA_RPPI ? __fmaf_rn(pif, P.xs, vxc)
                                 : __fmaf_rn(pif, __builtin_amdgcn_rsqf(rv32), 1.0f) * P.xs;
     int ir = (int)gr, i2 = (int)g2;
     bool near = ((int)(fabsf(gr - rintf(gr)) < P.dr) | (int)(fabsf(g2 - rintf(g2)) < P.d2)) != 0;
@@ -694,19 +694,18 @@ __device__ __forceinline__ void lc_pair(const LcPairArgs &P, bool valid, double 
         }
     }
     const int b = __mul24(ir, n_2) + i2;
-    if (b != pd.b) {
-        if (pd.b >= 0) lc_hist_add<SHAPES>(P, hist, pd);
-        pd.b = b;
-        pd.c = 0;
-        pd.dd = 0.0;
-        pd.sp = 0.0;
-        pd.sx = 0.0;
-    }
-    pd.c += 1;
-    pd.dd += pw * qw;
+    // a new slot flushes the pending run, which then restarts from this pair:
+    // keep = 0 drops the old sums (fma(0, sum, x) = x and fma(1, sum, x) =
+    // sum + x, exactly as the plain assignment and addition)
+    const bool nw = b != pd.b;
+    if (nw & (pd.b >= 0)) lc_hist_add<SHAPES>(P, hist, pd);
+    const double keep = nw ? 0.0 : 1.0;
+    pd.b = b;
+    pd.c = (nw ? 0u : pd.c) + 1u;
+    pd.dd = fma(keep, pd.dd, pw * qw);
     if (SHAPES) {
-        pd.sp += ep;
-        pd.sx += ex;
+        pd.sp = fma(keep, pd.sp, ep);
+        pd.sx = fma(keep, pd.sx, ex);
     }
 }
 
@@ -748,6 +747,11 @@ __global__ __launch_bounds__(kLcThreads) void k_lc_pairs(LcPairArgs P) {
     CPtrI ranges = (CPtrI)P.ranges;
     int cur_pn = -1, cur_ps = -1;
     uint32_t nre = 0;
+    // the additive constants of the FP32 bin guesses, held in VGPRs (a VOP3
+    // FMA reads one scalar operand; the other would be copied on every pair)
+    float vlgc, vxc;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(vlgc) : "s"(P.lgc));
+    asm volatile("v_mov_b32 %0, %1" : "=v"(vxc) : "s"(P.xc));
     for (int64_t i = it0; i < it1; ++i) {
         CPtrI ip = items + 4 * i;
         const int blk = ip[0], r0 = ip[1], nr = ip[2], ns = ip[3];
@@ -793,12 +797,12 @@ __global__ __launch_bounds__(kLcThreads) void k_lc_pairs(LcPairArgs P) {
                 const double b0 = r1[0], b1 = r1[1], b2 = r1[2], b3 = r1[3], b4 = r1[4], b5 = r1[5],
                              b6 = r1[6];
                 lc_pair<MODE, SHAPES>(P, valid, qra, qdec, qchi, qc2, qs2, qw, a0, a1, a2, a3, a4, a5,
-                                      a6, hist, T, X, nre, pd);
+                                      a6, hist, T, X, nre, pd, vlgc, vxc);
                 CPtrD r2 = pr + 8 * (k + 2 < np ? k + 2 : k + 1);
                 const double c0 = r2[0], c1 = r2[1], c2 = r2[2], c3 = r2[3], c4 = r2[4], c5 = r2[5],
                              c6 = r2[6];
                 lc_pair<MODE, SHAPES>(P, valid, qra, qdec, qchi, qc2, qs2, qw, b0, b1, b2, b3, b4, b5,
-                                      b6, hist, T, X, nre, pd);
+                                      b6, hist, T, X, nre, pd, vlgc, vxc);
                 a0 = c0;
                 a1 = c1;
                 a2 = c2;
@@ -809,7 +813,7 @@ __global__ __launch_bounds__(kLcThreads) void k_lc_pairs(LcPairArgs P) {
             }
             if (k < np)
                 lc_pair<MODE, SHAPES>(P, valid, qra, qdec, qchi, qc2, qs2, qw, a0, a1, a2, a3, a4, a5,
-                                      a6, hist, T, X, nre, pd);
+                                      a6, hist, T, X, nre, pd, vlgc, vxc);
         }
         if (pd.b >= 0) lc_hist_add<SHAPES>(P, hist, pd);
     }
