@@ -194,12 +194,26 @@ class MeasureIALightcone(SimInfo):
     _store = MeasureIABox._store
     _measure_w_g_i = MeasureIABox._measure_w_g_i
     _measure_multipoles = MeasureIABox._measure_multipoles
+    # covariance across datasets, inherited by the reference's lightcone class
+    # from MeasureJackknife (measure_jackknife.py:485-648); both read only the
+    # stored realisations
+    measure_covariance_multiple_datasets = MeasureIABox.measure_covariance_multiple_datasets
+    create_full_cov_matrix_projections = MeasureIABox.create_full_cov_matrix_projections
 
     # -- the reference's helpers --------------------------------------------------
     @staticmethod
     def get_ellipticity(e, phi):
         """e+ = e cos 2phi, ex = e sin 2phi (measure_IA_base.py:209-227)."""
         return e * np.cos(2 * phi), e * np.sin(2 * phi)
+
+    # the rest of MeasureIABase's helpers, which the reference's lightcone
+    # class inherits too (measure_IA_base.py:186-401)
+    calculate_dot_product_arrays = staticmethod(MeasureIABox.calculate_dot_product_arrays)
+    setdiff2D = staticmethod(MeasureIABox.setdiff2D)
+    setdiff_omit = staticmethod(MeasureIABox.setdiff_omit)
+    get_random_pairs = staticmethod(MeasureIABox.get_random_pairs)
+    get_volume_spherical_cap = staticmethod(MeasureIABox.get_volume_spherical_cap)
+    get_random_pairs_r_mur = MeasureIABox.get_random_pairs_r_mur
 
     def assign_jackknife_patches(self, data, randoms_data, num_jk):
         """k-means patches on (RA, DEC) of the randoms (measure_IA_base.py:745-800).

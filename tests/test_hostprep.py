@@ -113,3 +113,22 @@ def test_front_end_construction_and_validation():
         box.measure_xi_w("x", "both", 9, temp_file_path=None)
     mixed = MeasureIABox(None, None, "TNG300", 99, verbose=False)
     assert mixed.Num_position == 0
+
+
+def test_lightcone_class_has_the_base_helpers():
+    """MeasureIALightcone carries MeasureIABase's helpers, as the reference's
+    lightcone class inherits them (measure_IA_base.py:186-401)."""
+    from measure_ia_amd import MeasureIABox, MeasureIALightcone
+    cols = {k: np.zeros(2) for k in ("RA", "DEC", "Redshift", "RA_shape_sample", "DEC_shape_sample",
+                                     "Redshift_shape_sample", "e1", "e2")}
+    lc = MeasureIALightcone(cols, {k: np.zeros(2) for k in ("RA", "DEC", "Redshift")}, None,
+                            separation_limits=[0.1, 20.0], num_bins_r=4, num_bins_pi=4, pi_max=10.0)
+    a = np.arange(6.0).reshape(3, 2)
+    np.testing.assert_array_equal(lc.calculate_dot_product_arrays(a, a), [1.0, 13.0, 41.0])
+    assert [list(x) for x in lc.setdiff2D([[1, 2, 3], [4, 5]], [[2], [5]])] == [[1, 3], [4]]
+    assert [list(x) for x in lc.setdiff_omit([[1, 2], [3, 4]], [2, 3], [1])] == [[4]]
+    assert lc.get_random_pairs(2.0, 1.0, 5.0, -5.0, 1000.0, "cross", 10, 4) == \
+        MeasureIABox.get_random_pairs(2.0, 1.0, 5.0, -5.0, 1000.0, "cross", 10, 4)
+    assert lc.get_volume_spherical_cap(0.5, 2.0) == MeasureIABox.get_volume_spherical_cap(0.5, 2.0)
+    v = lc.get_random_pairs_r_mur(2.0, 1.0, 1.0, -1.0, 1000.0, "cross", 10, 4)
+    assert v == pytest.approx(10 * 4 * (4 / 3 * np.pi * (8 - 1)) / 1000.0 * 9 / 10, rel=1e-12)

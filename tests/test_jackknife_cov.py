@@ -81,3 +81,27 @@ def test_full_cov_matrix_projections(tmp_path):
         np.testing.assert_array_equal(
             st.read("Snapshot_99/w_g_plus/set1_set2_set3_combined_jackknife_cov_3"), cov3)
         np.testing.assert_array_equal(st.read("Snapshot_99/w_g_plus/set1_set2_jackknife_cov_3"), c12[0:4, 4:8])
+
+
+def test_lightcone_class_has_the_covariance_methods(tmp_path):
+    """The reference's MeasureIALightcone inherits both methods from
+    MeasureJackknife; lightcone outputs carry no snapshot group."""
+    from measure_ia_amd import MeasureIALightcone
+    fn = str(tmp_path / "lc_cov.hdf5")
+    with H5Store(fn, "a") as st:
+        for ds in ("set1", "set2", "set3"):
+            for b in range(3):
+                st.write(f"w_g_plus/{ds}_jk3", f"{ds}_{b}", MEANS[ds] + DEV[ds][b])
+    cols = {k: np.zeros(2) for k in ("RA", "DEC", "Redshift", "RA_shape_sample", "DEC_shape_sample",
+                                     "Redshift_shape_sample", "e1", "e2")}
+    lc = MeasureIALightcone(cols, {k: np.zeros(2) for k in ("RA", "DEC", "Redshift")}, fn,
+                            separation_limits=[0.1, 20.0], num_bins_r=4, num_bins_pi=4, pi_max=10.0)
+    cov1 = lc.measure_covariance_multiple_datasets("w_g_plus", ["set1"], 3, True)[0]
+    np.testing.assert_allclose(
+        cov1, 2 / 3 * np.array([[2, 1, 1, -1], [1, 2, 2, -2], [1, 2, 2, -2], [-1, -2, -2, 2]]),
+        rtol=1e-15, atol=1e-15)
+    for ds in ("set1", "set2", "set3"):
+        lc.measure_covariance_multiple_datasets("w_g_plus", [ds], 3, False)
+    cov3, c12, _, c23 = lc.create_full_cov_matrix_projections("w_g_plus", ["set1", "set2", "set3"], 3, True)
+    np.testing.assert_array_equal(cov3[0:8, 0:8], c12)
+    np.testing.assert_array_equal(cov3[4:12, 4:12], c23)
