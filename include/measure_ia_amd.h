@@ -227,7 +227,7 @@ int mia_count_lightcone(mia_ctx *ctx, const mia_lc_sample *sample, const mia_bin
 
 /* Jackknife region id of each object (measure_IA_base.py:404-452): strict
  * i*L_sub < x < (i+1)*L_sub on every axis, region i*L^2 + j*L + k, else 0.
- * xyz device n x 3, region device n int32. */
+ * xyz device n x 3, region device n int32; 1 <= jk_per_side <= 1024. */
 int mia_region_ids(mia_ctx *ctx, const double *xyz, int64_t n, double jk_sub,
                    int32_t jk_per_side, int32_t *region, void *stream);
 

@@ -242,6 +242,8 @@ int mia_region_ids(mia_ctx *ctx, const double *xyz, int64_t n, double jk_sub, in
     return guard(ctx, [&] {
         MIA_REQUIRE(n >= 0 && (n == 0 || (xyz && region)), "bad arrays");
         MIA_REQUIRE(jk_per_side >= 1 && jk_sub > 0, "bad jackknife split");
+        // region ids i*L^2 + j*L + k must fit int32
+        MIA_REQUIRE(jk_per_side <= 1024, "jk_per_side above 1024 (region ids would overflow int32)");
         MIA_HIP(hipSetDevice(ctx->c.device));
         mia::region_ids(ctx->c, xyz, n, jk_sub, jk_per_side, region, (hipStream_t)stream);
         MIA_HIP(hipStreamSynchronize((hipStream_t)stream));

@@ -68,6 +68,12 @@ def test_bad_arguments_are_refused(engine):
     assert e.value.code == -1
     ok = engine.count(_sample(pos, shp), _th())
     assert ok.npairs[0].sum() > 0
+    # region ids: a split whose ids would overflow int32, and no split at all
+    for jk in (1025, 0):
+        with pytest.raises(MiaError) as e:
+            engine.region_ids(pos, jk, L / max(jk, 1))
+        assert e.value.code == -1
+    np.testing.assert_array_equal(engine.region_ids(pos[:3], 1, L), [0, 0, 0])
 
 
 def test_empty_samples_give_zero_counts(engine):
