@@ -380,7 +380,7 @@ def main():
                      s_per_full_count_est=pairs / plain_rate, kernel_ms=rp.stats["ms_pairs"])
         log(f"plain: {plain_rate:.3e} pairs/s on {frac:.2%} of shapes")
 
-    e2e = None
+    e2e = e2e_parts = None
     if not a.no_e2e and not batch and w["mode"] == "rppi":
         from measure_ia_amd import MeasureIABox
         with tempfile.TemporaryDirectory() as td:
@@ -391,7 +391,9 @@ def main():
             te = time.perf_counter()
             box.measure_xi_w("bench", "both", jk ** 3, temp_file_path=None)
             e2e = time.perf_counter() - te
-        log(f"end-to-end measure_xi_w: {e2e:.3f} s")
+            # host prep + H2D copy, device count, estimators, HDF5 writes
+            e2e_parts = {k: round(v, 5) for k, v in box.last_timing.items() if k.endswith("_s")}
+        log(f"end-to-end measure_xi_w: {e2e:.3f} s {e2e_parts}")
 
     cpu = None
     if not a.no_cpu:
@@ -432,6 +434,7 @@ def main():
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
         "s_per_wgg_wgplus": ms_step / 1e3,
         "end_to_end_measure_xi_w_s": e2e,
+        "end_to_end_parts_s": e2e_parts,
         "engine": dict(st),
     }
     print(json.dumps(line))
