@@ -4,8 +4,6 @@ Every dataset the reference would write must be present (and nothing else),
 with gg quantities (DD, RR, xi_gg, w_gg, their realisations and covariance)
 bit-exact for unit weights and every g+ / gx quantity within 1e-6.
 """
-import os
-
 import numpy as np
 import pytest
 
