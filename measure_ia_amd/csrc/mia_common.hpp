@@ -16,6 +16,16 @@
 namespace mia {
 
 constexpr int kWave = 64;
+constexpr int kXcds = 8;            // accelerator complex dies, each with its own L2
+
+// Workgroup number in XCD-major order.  Workgroups are dispatched round-robin
+// over the XCDs, so with this numbering the workgroups of one XCD take one
+// contiguous share of a sorted work list (and its shared inputs stay in that
+// XCD's L2).  A grid whose size is not a multiple of kXcds keeps its order.
+__device__ __forceinline__ int xcd_major_block() {
+    const int g = (int)gridDim.x, b = (int)blockIdx.x;
+    return g % kXcds == 0 ? (b % kXcds) * (g / kXcds) + b / kXcds : b;
+}
 constexpr int kMaxBins2 = 256;      // max n_2 (pi or mu bins)
 constexpr int kMaxBinsR = 256;      // max n_r
 constexpr int kMaxBlockShapes = 64;
