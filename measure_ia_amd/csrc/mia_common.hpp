@@ -18,13 +18,22 @@ namespace mia {
 constexpr int kWave = 64;
 constexpr int kXcds = 8;            // accelerator complex dies, each with its own L2
 
-// Workgroup number in XCD-major order.  Workgroups are dispatched round-robin
-// over the XCDs, so with this numbering the workgroups of one XCD take one
-// contiguous share of a sorted work list (and its shared inputs stay in that
-// XCD's L2).  A grid whose size is not a multiple of kXcds keeps its order.
-__device__ __forceinline__ int xcd_major_block() {
+// Workgroup number that keeps runs of consecutive work on one XCD.
+// Workgroups are dispatched round-robin over the XCDs, so hardware workgroups
+// b, b + 8, b + 16, ... share an L2.  The logical numbering gives each XCD
+// runs of S = 16 consecutive workgroups (64 waves) in turn: neighbouring
+// items of a sorted work list, which read the same inputs, meet in one L2,
+// while every XCD still sees every part of the list (one contiguous eighth
+// per XCD left whole samples of a batched launch to single XCDs and
+// unbalanced them).  S halves until it divides the grid; a grid that is not
+// a multiple of kXcds keeps its order.
+__device__ __forceinline__ int xcd_run_block() {
     const int g = (int)gridDim.x, b = (int)blockIdx.x;
-    return g % kXcds == 0 ? (b % kXcds) * (g / kXcds) + b / kXcds : b;
+    int S = 16;
+    while (S > 1 && g % (kXcds * S) != 0) S >>= 1;
+    if (g % (kXcds * S) != 0) return b;
+    const int slot = b / kXcds, x = b % kXcds;
+    return ((slot / S) * kXcds + x) * S + slot % S;
 }
 constexpr int kMaxBins2 = 256;      // max n_2 (pi or mu bins)
 constexpr int kMaxBinsR = 256;      // max n_r

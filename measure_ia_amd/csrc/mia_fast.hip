@@ -696,9 +696,9 @@ __global__ __launch_bounds__(kThreads) void k_pairs(PairArgs P) {
     for (int i = lane * 4; i < P.hist_stride; i += kWave * 4) *(uint32_t *)(hist + i) = 0u;
     __syncthreads();
 
-    // XCD-major numbering: neighbouring items of the key-sorted list read the
-    // same position cells, and one XCD's waves take a contiguous share of them
-    const int w = xcd_major_block() * (blockDim.x / kWave) + wv;
+    // runs of neighbouring items of the key-sorted list (which read the same
+    // position cells) stay on one XCD
+    const int w = xcd_run_block() * (blockDim.x / kWave) + wv;
     if (w >= P.n_waves || P.n_items == 0) return;
     const MIA_CONST int64_t *cpre = (const MIA_CONST int64_t *)P.cost_prefix;
     const double total_cost = (double)*(const MIA_CONST int64_t *)P.total_cost;

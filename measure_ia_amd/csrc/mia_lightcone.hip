@@ -724,9 +724,9 @@ __global__ __launch_bounds__(kLcThreads) void k_lc_pairs(LcPairArgs P) {
     for (int i = lane * 4; i < P.hist_stride; i += kWave * 4) *(uint32_t *)(hist + i) = 0u;
     __syncthreads();
 
-    // XCD-major numbering: one XCD's waves take a contiguous share of the
-    // patch-sorted item list, whose neighbouring items read the same blocks
-    const int w = xcd_major_block() * (blockDim.x / kWave) + wv;
+    // runs of neighbouring items of the patch-sorted list (which read the
+    // same position blocks) stay on one XCD
+    const int w = xcd_run_block() * (blockDim.x / kWave) + wv;
     if (w >= P.n_waves || P.n_items == 0) return;
     const __attribute__((address_space(4))) int64_t *cpre =
         (const __attribute__((address_space(4))) int64_t *)P.cost_prefix;
