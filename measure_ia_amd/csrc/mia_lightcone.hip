@@ -784,6 +784,43 @@ __global__ __launch_bounds__(kLcThreads) void k_lc_pairs(LcPairArgs P) {
                     o = 0;
                 }
             }
+            if (filled <= kWave / 2 && np > 1) {
+                // Few shapes: R replicas of the chunk's shapes fill the
+                // lanes, replica r taking positions r, r + R, ... of the
+                // block, whose records are then read per lane.
+                const int R = kWave / filled < np ? kWave / filled : np;
+                const int rep = lane / filled;
+                const int msh = __shfl(my, lane - rep * filled);
+                const bool sv = (rep < R) & (msh >= 0);
+                const int q = sv ? msh : 0;
+                const double qra = P.sra[q], qdec = P.sdec[q], qchi = P.schi[q], qw = P.sw[q];
+                const double qc2 = SHAPES ? P.sc2w[q] : 0.0, qs2 = SHAPES ? P.ss2w[q] : 0.0;
+                int k = rep;
+                bool v = sv & (k < np);
+                const double *rec = P.prec + 8 * ((int64_t)p0 + (v ? k : 0));
+                double a0 = rec[0], a1 = rec[1], a2 = rec[2], a3 = rec[3], a4 = rec[4], a5 = rec[5],
+                       a6 = rec[6];
+                const int nt = (np + R - 1) / R;
+                for (int t = 0; t < nt; ++t) {
+                    const int kn = k + R;
+                    const bool vn = sv & (kn < np);
+                    const double *rn = P.prec + 8 * ((int64_t)p0 + (vn ? kn : 0));
+                    const double b0 = rn[0], b1 = rn[1], b2 = rn[2], b3 = rn[3], b4 = rn[4], b5 = rn[5],
+                                 b6 = rn[6];
+                    lc_pair<MODE, SHAPES>(P, v, qra, qdec, qchi, qc2, qs2, qw, a0, a1, a2, a3, a4, a5, a6,
+                                          hist, T, X, nre, pd, vlgc, vxc);
+                    a0 = b0;
+                    a1 = b1;
+                    a2 = b2;
+                    a3 = b3;
+                    a4 = b4;
+                    a5 = b5;
+                    a6 = b6;
+                    v = vn;
+                    k = kn;
+                }
+                continue;
+            }
             const bool valid = my >= 0;
             const int q = valid ? my : 0;
             const double qra = P.sra[q], qdec = P.sdec[q], qchi = P.schi[q], qw = P.sw[q];
