@@ -111,7 +111,7 @@ __global__ __launch_bounds__(256) void k_lc_bounds(const double *__restrict__ ra
                                                    const double *__restrict__ dec,
                                                    const double *__restrict__ chi,
                                                    const int32_t *__restrict__ patch, int64_t n,
-                                                   int P, int sum_chi, LcBounds *out) {
+                                                   int P, int sum_chi, LcBounds *part) {
     __shared__ unsigned long long s_lo[3][256], s_hi[3][256];
     __shared__ double s_sum[256];
     __shared__ int s_bad;
@@ -149,15 +149,74 @@ __global__ __launch_bounds__(256) void k_lc_bounds(const double *__restrict__ ra
         }
         __syncthreads();
     }
+    // one partial per workgroup (reduced by k_lc_bounds_reduce: no contended
+    // global atomics, and the chi sum has a fixed order)
     if (threadIdx.x == 0) {
-        atomicMin(&out->ra_lo, s_lo[0][0]);
-        atomicMax(&out->ra_hi, s_hi[0][0]);
-        atomicMin(&out->dec_lo, s_lo[1][0]);
-        atomicMax(&out->dec_hi, s_hi[1][0]);
-        atomicMin(&out->chi_lo, s_lo[2][0]);
-        atomicMax(&out->chi_hi, s_hi[2][0]);
-        if (sum_chi) atomicAdd(&out->chi_sum, s_sum[0]);
-        if (s_bad) atomicOr(&out->bad, s_bad);
+        LcBounds r;
+        r.ra_lo = s_lo[0][0];
+        r.ra_hi = s_hi[0][0];
+        r.dec_lo = s_lo[1][0];
+        r.dec_hi = s_hi[1][0];
+        r.chi_lo = s_lo[2][0];
+        r.chi_hi = s_hi[2][0];
+        r.chi_sum = sum_chi ? s_sum[0] : 0.0;
+        r.bad = s_bad;
+        r.pad = 0;
+        part[blockIdx.x] = r;
+    }
+}
+
+// The partials of k_lc_bounds into one LcBounds (one workgroup).
+__global__ __launch_bounds__(256) void k_lc_bounds_reduce(const LcBounds *__restrict__ part, int n,
+                                                          LcBounds *__restrict__ out) {
+    __shared__ unsigned long long s_lo[3][256], s_hi[3][256];
+    __shared__ double s_sum[256];
+    __shared__ int s_bad[256];
+    unsigned long long lo[3] = {~0ull, ~0ull, ~0ull}, hi[3] = {0ull, 0ull, 0ull};
+    double sum = 0.0;
+    int bad = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const LcBounds p = part[i];
+        const unsigned long long plo[3] = {p.ra_lo, p.dec_lo, p.chi_lo};
+        const unsigned long long phi[3] = {p.ra_hi, p.dec_hi, p.chi_hi};
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = plo[k] < lo[k] ? plo[k] : lo[k];
+            hi[k] = phi[k] > hi[k] ? phi[k] : hi[k];
+        }
+        sum += p.chi_sum;
+        bad |= p.bad;
+    }
+    for (int k = 0; k < 3; ++k) {
+        s_lo[k][threadIdx.x] = lo[k];
+        s_hi[k][threadIdx.x] = hi[k];
+    }
+    s_sum[threadIdx.x] = sum;
+    s_bad[threadIdx.x] = bad;
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) {
+            for (int k = 0; k < 3; ++k) {
+                unsigned long long a = s_lo[k][threadIdx.x + st], b = s_hi[k][threadIdx.x + st];
+                if (a < s_lo[k][threadIdx.x]) s_lo[k][threadIdx.x] = a;
+                if (b > s_hi[k][threadIdx.x]) s_hi[k][threadIdx.x] = b;
+            }
+            s_sum[threadIdx.x] += s_sum[threadIdx.x + st];
+            s_bad[threadIdx.x] |= s_bad[threadIdx.x + st];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        LcBounds r;
+        r.ra_lo = s_lo[0][0];
+        r.ra_hi = s_hi[0][0];
+        r.dec_lo = s_lo[1][0];
+        r.dec_hi = s_hi[1][0];
+        r.chi_lo = s_lo[2][0];
+        r.chi_hi = s_hi[2][0];
+        r.chi_sum = s_sum[0];
+        r.bad = s_bad[0];
+        r.pad = 0;
+        *out = r;
     }
 }
 
@@ -971,17 +1030,21 @@ void count_lightcone(Ctx &ctx, const mia_lc_sample &s, const mia_bins &bins, con
 
     // ---- 1. bounds (one readback)
     LcBounds *bd = ctx.arena.get<LcBounds>("lc/bounds", 2);
-    LcBounds init{};
-    init.ra_lo = init.dec_lo = init.chi_lo = ~0ull;
-    init.ra_hi = init.dec_hi = init.chi_hi = 0ull;
-    LcBounds hinit[2] = {init, init};
-    MIA_HIP(hipMemcpyAsync(bd, hinit, sizeof(hinit), hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_lc_bounds, dim3(std::min<unsigned>(nblk_(np_, 256), 1024)), dim3(256), 0, st,
-                       s.ra, s.dec, s.chi, s.patch, np_, P, 1, bd);
-    MIA_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_lc_bounds, dim3(std::min<unsigned>(nblk_(ns_, 256), 1024)), dim3(256), 0, st,
-                       s.ra_s, s.dec_s, s.chi_s, s.patch_s, ns_, P, 0, bd + 1);
-    MIA_HIP(hipGetLastError());
+    {
+        const unsigned gp = std::min<unsigned>(nblk_(np_, 256), 1024);
+        const unsigned gs = std::min<unsigned>(nblk_(ns_, 256), 1024);
+        LcBounds *part = ctx.arena.get<LcBounds>("lc/bounds_part", (size_t)gp + gs);
+        hipLaunchKernelGGL(k_lc_bounds, dim3(gp), dim3(256), 0, st, s.ra, s.dec, s.chi, s.patch, np_, P, 1,
+                           part);
+        MIA_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_lc_bounds, dim3(gs), dim3(256), 0, st, s.ra_s, s.dec_s, s.chi_s, s.patch_s, ns_,
+                           P, 0, part + gp);
+        MIA_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_lc_bounds_reduce, dim3(1), dim3(256), 0, st, part, (int)gp, bd);
+        MIA_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_lc_bounds_reduce, dim3(1), dim3(256), 0, st, part + gp, (int)gs, bd + 1);
+        MIA_HIP(hipGetLastError());
+    }
     LcBounds hb[2];
     MIA_HIP(hipMemcpyAsync(hb, bd, sizeof(hb), hipMemcpyDeviceToHost, st));
     MIA_HIP(hipStreamSynchronize(st));
