@@ -1116,7 +1116,12 @@ void count_lightcone(Ctx &ctx, const mia_lc_sample &s, const mia_bins &bins, con
         const double ncell = (double)std::max(1, std::min(nrows, 4096)) * std::max(1, std::min(ncols, 4096));
         const double nsl4 = std::floor((pchi_hi - pchi_lo) / std::fmax((g.whi - g.wlo) * 0.5, 1e-9)) + 1;
         const double per_seg = (double)np_ / (ncell * nsl4);
-        eighths = (int)std::lround(4.0 * std::sqrt(16.0 / std::fmax(per_seg, 1e-3)));
+        // ~16 positions per segment, deeper where shapes are sparse next to
+        // positions: a block's per-item cost is then spread over more pairs
+        // (measured on 1M positions / 300k shapes: 8 eighths 4 % faster than 5;
+        // equal samples keep the plain rule)
+        const double sparse = std::sqrt(std::fmax((double)np_ / (double)ns_, 1.0));
+        eighths = (int)std::lround(4.0 * std::sqrt(16.0 / std::fmax(per_seg, 1e-3)) * sparse);
         eighths = std::max(2, std::min(eighths, 8));
     }
     if (tuning && tuning->slabs_per_block > 0) eighths = tuning->slabs_per_block;
