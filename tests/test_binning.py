@@ -3,10 +3,12 @@
 The reference formulas (measure_w_box_jk.py:142-143, 178-193;
 measure_m_box_jk.py:149-150, 187-204) are evaluated directly with numpy on
 random values plus every threshold and its neighbouring doubles; the
-threshold rule (count of thresholds <= value) must agree everywhere.
+threshold rule (count of thresholds <= value) must agree everywhere, on the
+benchmark binnings and on 40 random ones.
 """
 import numpy as np
 import pytest
+from hypothesis import given, settings, strategies as st
 
 from measure_ia_amd import binning as B
 from oracle import prep
@@ -89,3 +91,40 @@ def test_planted_log10_slip_is_caught():
     s = th.r2_edges[3]
     assert np.searchsorted(bad[1:-1], s, side="right") != np.searchsorted(th.r2_edges[1:-1], s,
                                                                           side="right")
+
+
+
+@settings(max_examples=40, deadline=None, derandomize=True)
+@given(rmin=st.floats(0.01, 5.0), ratio=st.floats(1.5, 1000.0), nr=st.integers(1, 60),
+       n2=st.integers(1, 60), pimax=st.floats(0.5, 300.0))
+def test_random_binnings_match_formula(rmin, ratio, nr, n2, pimax):
+    """Random binnings (the sizes a user may choose): the threshold rule equals
+    the reference's index formulas at random values and around every threshold,
+    for (r_p, pi) and for mu."""
+    rmax = rmin * ratio
+    r_bins, pi_bins, mu_bins = prep.bin_edges(rmin, rmax, nr, n2, pimax, 2 * pimax)
+    rng = np.random.default_rng(nr * 1000 + n2)
+    th = B.rppi_thresholds(r_bins, pi_bins, rmin, rmax)
+    s = np.exp(rng.uniform(np.log(r_bins[0] ** 2 * 0.9), np.log(r_bins[-1] ** 2 * 1.1), 20_000))
+    s = np.concatenate([s, _neigh(th.r2_edges), _neigh(r_bins ** 2)])
+    pi = rng.uniform(pi_bins[0] * 1.1, pi_bins[-1] * 1.1, len(s))
+    pi[: 3 * len(th.x2_edges)] = _neigh(th.x2_edges)
+    r = np.sqrt(s)
+    dom, ir, ip = B.bin_rppi(s, pi, th)
+    np.testing.assert_array_equal(dom, (r >= r_bins[0]) & (r < r_bins[-1]) & (pi >= pi_bins[0])
+                                  & (pi < pi_bins[-1]))
+    irr = np.array(B.r_index(s[dom], r_bins, rmin, rmax), dtype=int)
+    irr[irr >= nr] = nr - 1
+    step = (pi_bins[-1] - pi_bins[0]) / n2
+    ipr = np.array(np.floor(pi[dom] / step - pi_bins[0] / step), dtype=int)
+    ipr[ipr >= n2] = n2 - 1
+    np.testing.assert_array_equal(ir[dom], irr)
+    np.testing.assert_array_equal(ip[dom], ipr)
+    thm = B.rmu_thresholds(r_bins, mu_bins, rmin, rmax)
+    mu = np.concatenate([rng.uniform(-1, 1, 20_000), _neigh(thm.x2_edges), _neigh([-1.0, 1.0])])
+    _, _, im = B.bin_rmu(np.full(len(mu), thm.r2_edges[0]), mu, thm)
+    mstep = 2.0 / n2
+    imr = np.array(np.floor(mu / mstep - mu_bins[0] / mstep), dtype=int)
+    imr[imr >= n2] = n2 - 1
+    imr = np.where(imr < 0, imr + n2, imr)
+    np.testing.assert_array_equal(im, imr)
