@@ -20,7 +20,8 @@
 // mu = pi / r, and the scaled r_p only enters the rp_cut test).
 //
 // Device steps:
-//   1. bounds : min/max of ra, dec, chi (one readback) -> the grid.
+//   1. bounds : min/max of ra, dec, chi (per-workgroup partials, one
+//               reducing workgroup, one readback) -> the grid.
 //   2. sort   : positions by (cell, patch, chi slab), shapes by (cell, patch),
 //               chi ascending inside (two stable radix sorts), gathered into
 //               position records and shape arrays.
@@ -33,7 +34,10 @@
 //   4. pairs  : persistent waves over the cost-balanced items.  A batch of 64
 //               lanes is filled from the item's ranges (one shape per lane);
 //               the block's positions are read through the scalar cache, one
-//               at a time, and every lane evaluates its pair.  Separations are
+//               at a time, and every lane evaluates its pair.  A batch of at
+//               most 32 shapes is replicated over the lanes instead, each
+//               replica taking every R-th position of the block (records read
+//               per lane).  Separations are
 //               computed with the position's precomputed scale
 //               (pi_c/180 * chi_n * cos dec_n), bins are guessed in FP32 and,
 //               within a guard band of any edge, the pair is recomputed in the
