@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define MIA_ABI_VERSION 1
+#define MIA_ABI_VERSION 2
 
 enum mia_status {
     MIA_OK = 0,
@@ -135,6 +135,9 @@ typedef struct mia_tuning {
     int32_t hist_copies;        /* interleaved LDS copies of each histogram bin (1,2,4) */
     int32_t cell_shapes;        /* > 0: size cells to hold this many shapes on average
                                    instead of the cost-model search                     */
+    int32_t r_bins_per_pass;    /* > 0: r bins per pass of the pair kernel (a binning
+                                   whose wave histograms would leave fewer than 8 waves
+                                   per CU is counted in passes over r-bin ranges)      */
 } mia_tuning;
 
 /* Per-call statistics, filled when the pointer is non-NULL. */

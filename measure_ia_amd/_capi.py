@@ -58,7 +58,8 @@ class MiaTuning(ctypes.Structure):
     _fields_ = [("columns_per_side", ctypes.c_int32), ("slabs_per_side", ctypes.c_int32),
                 ("slabs_per_block", ctypes.c_int32), ("block_shapes", ctypes.c_int32),
                 ("item_positions", ctypes.c_int32), ("waves", ctypes.c_int32),
-                ("hist_copies", ctypes.c_int32), ("cell_shapes", ctypes.c_int32)]
+                ("hist_copies", ctypes.c_int32), ("cell_shapes", ctypes.c_int32),
+                ("r_bins_per_pass", ctypes.c_int32)]
 
 
 class MiaStats(ctypes.Structure):

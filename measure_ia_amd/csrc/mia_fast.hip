@@ -363,6 +363,7 @@ struct PairArgs {
     int scan;                    // 1: the exact path scans every threshold (guard band of a whole bin)
     const SampleDev *samples;
     int hist_stride;             // bytes per wave histogram
+    int bin_off, nbins_out;      // this pass's first output bin (r0 * n_2) and the output's B
     int copies_log2;             // each bin is kept in 2^copies_log2 interleaved copies
     int off_sp, off_sx, off_dd, off_var; // byte offsets inside a wave histogram
     unsigned long long *rechecks;
@@ -389,6 +390,7 @@ __device__ void flush_hist(const PairArgs &P, char *hist, const SampleDev &S, in
     double *dd = P.weighted ? (double *)(hist + P.off_dd) : nullptr;
     double *vr = P.var ? (double *)(hist + P.off_var) : nullptr;
     const int B = P.nbins;
+    const int BO = P.nbins_out;
     const int C = 1 << P.copies_log2;
     for (int b = lane; b < B; b += kWave) {
         uint32_t c = 0;
@@ -411,18 +413,19 @@ __device__ void flush_hist(const PairArgs &P, char *hist, const SampleDev &S, in
             }
         }
         if (c == 0) continue;
-        atomicAdd(&S.npairs[b], (unsigned long long)c);
-        atomicAdd(&S.sp[b], vsp);
-        atomicAdd(&S.sx[b], vsx);
-        if (dd && S.dd) atomicAdd(&S.dd[b], vdd);
-        if (vr && S.var) atomicAdd(&S.var[b], vvr);
+        const int gb = b + P.bin_off;  // the pass's local bin in the output grid
+        atomicAdd(&S.npairs[gb], (unsigned long long)c);
+        atomicAdd(&S.sp[gb], vsp);
+        atomicAdd(&S.sx[gb], vsx);
+        if (dd && S.dd) atomicAdd(&S.dd[gb], vdd);
+        if (vr && S.var) atomicAdd(&S.var[gb], vvr);
         if (S.K > 0) {
-            int o = (1 + rs) * B + b;
+            int o = (1 + rs) * BO + gb;
             atomicAdd(&S.npairs[o], (unsigned long long)c);
             atomicAdd(&S.sp[o], vsp);
             if (dd && S.dd) atomicAdd(&S.dd[o], vdd);
             if (rp != rs) {
-                o = (1 + rp) * B + b;
+                o = (1 + rp) * BO + gb;
                 atomicAdd(&S.npairs[o], (unsigned long long)c);
                 atomicAdd(&S.sp[o], vsp);
                 if (dd && S.dd) atomicAdd(&S.dd[o], vdd);
@@ -798,7 +801,10 @@ inline unsigned nblk(int64_t n, int t) { return (unsigned)((n + t - 1) / t < 1 ?
 bool fast_path_fits(const mia_bins &bins, bool weighted, int jk) {
     const int64_t K1 = jk > 0 ? (int64_t)jk * jk * jk : 1;
     if (K1 * K1 > kMaxKeysPerSample) return false;
-    return threshold_bytes(bins) + wave_hist_bytes(bins, weighted) <= kLdsBytes;
+    // larger binnings run in passes over r-bin ranges: one r row must fit
+    mia_bins row = bins;
+    row.n_r = 1;
+    return threshold_bytes(bins) + wave_hist_bytes(row, weighted) <= kLdsBytes;
 }
 
 void count_fast(Ctx &ctx, std::vector<FastSample> &samples, const mia_bins &bins,
@@ -1016,7 +1022,8 @@ void count_fast(Ctx &ctx, std::vector<FastSample> &samples, const mia_bins &bins
     A.lgs = (float)lgs;
     A.lgc = (float)(-l0 * lgs);
     const double u = std::ldexp(1.0, -23);  // FP32 ulp of 1
-    double err_r = (4 * u * std::fmax(std::fabs(l0), std::fabs(ln)) + 4 * u) * lgs + 2 * u * n_r;
+    // (+ 2u n_r for a pass's shifted constant -l0 lgs - r0)
+    double err_r = (4 * u * std::fmax(std::fabs(l0), std::fabs(ln)) + 4 * u) * lgs + 4 * u * n_r;
     double err_2;
     if (bins.mode == MIA_RPPI) {
         A.X0 = bins.x2_edges[0];
@@ -1040,19 +1047,69 @@ void count_fast(Ctx &ctx, std::vector<FastSample> &samples, const mia_bins &bins
     A.d2 = exact_only ? 1.0f : (float)d2;
     A.scan = exact_only ? 1 : 0;
     A.samples = sdd;
+    // Passes over r-bin ranges.  Pass [r0, r0 + nrp) takes the pairs with
+    // T[r0] <= r^2 < T[r0 + nrp] (exact FP64 comparisons, so every pair lands
+    // in one pass), bins them locally and flushes to output bins
+    // r0 * n_2 + local.  A pass's time goes as 1 / min(1, w / 16), w the
+    // waves per CU its histogram leaves room for (latency-bound below 16
+    // waves, VALU-bound above; measured on 60 x 60 and 100 x 100 bins), so nrp
+    // minimises passes / min(1, w / 16) (ties: fewer passes), with at least
+    // 4 waves per CU where any split allows it (at 2 a pass runs slower than
+    // that model says).  Ordinary binnings run in one pass.
+    const size_t thr_bytes = threshold_bytes(bins);
+    int nrp = n_r;
+    {
+        mia_bins part = bins;
+        part.accumulate_variance = any_var ? 1 : 0;
+        auto waves_per_cu = [&](int nr_) {
+            part.n_r = nr_;
+            const size_t hs = wave_hist_bytes(part, any_w);
+            int w = kMaxWavesPerBlock;  // as the launch below: 4, 2 or 1 waves per workgroup
+            while (w > 1 && thr_bytes + hs * w > kLdsBytes) w /= 2;
+            const size_t l = thr_bytes + hs * w;
+            if (l > kLdsBytes) return 0;
+            return (int)std::min<size_t>(32 / w, kLdsBytes / l) * w;
+        };
+        if (tuning && tuning->r_bins_per_pass > 0) {
+            // as asked, but no more than one wave's histogram can hold
+            nrp = std::min(n_r, tuning->r_bins_per_pass);
+            while (nrp > 1 && waves_per_cu(nrp) == 0) --nrp;
+        } else {
+            double best = 1e300;
+            int best_p = 0;
+            for (int pass_min_waves : {4, 1}) {
+                for (int c = n_r; c >= 1; --c) {
+                    const int w = waves_per_cu(c);
+                    if (w < pass_min_waves) continue;
+                    const int passes = (n_r + c - 1) / c;
+                    const double cost = (double)passes / std::min(1.0, w / 16.0);
+                    if (cost < best * (1 - 1e-9) || (cost <= best * (1 + 1e-9) && passes < best_p)) {
+                        best = cost;
+                        best_p = passes;
+                        nrp = c;
+                    }
+                }
+                if (best_p) break;
+            }
+        }
+    }
+    const int npass = (n_r + nrp - 1) / nrp;
+    const int Bp = nrp * n_2;     // bins of the largest pass
+    A.nbins_out = B;
+    A.bin_off = 0;
     // histogram copies (default 1: the pending-slot registers already absorb
     // most collisions, and a single copy leaves LDS for 8 waves per SIMD);
     // more copies only while LDS still holds ~5 waves per SIMD, at most 4
     int copies_log2 = 0;
     {
-        size_t per_copy = (size_t)B * (4 + 16 + (any_w ? 8 : 0) + (any_var ? 8 : 0));
+        size_t per_copy = (size_t)Bp * (4 + 16 + (any_w ? 8 : 0) + (any_var ? 8 : 0));
         int want = tuning && tuning->hist_copies > 0 ? tuning->hist_copies : 1;
         while ((1 << (copies_log2 + 1)) <= want && (1 << (copies_log2 + 1)) <= 4 &&
                per_copy * (2u << copies_log2) * 20 <= 160 * 1024)
             ++copies_log2;
     }
     A.copies_log2 = copies_log2;
-    const int BC = B << copies_log2;
+    const int BC = Bp << copies_log2;
     int off = 4 * BC;
     off = (off + 15) & ~15;
     A.off_sp = off;
@@ -1066,7 +1123,6 @@ void count_fast(Ctx &ctx, std::vector<FastSample> &samples, const mia_bins &bins
     A.hist_stride = (off + 15) & ~15;
     A.rechecks = rechecks;
     // waves per block: 4, or fewer when the histograms of 4 waves do not fit
-    const size_t thr_bytes = threshold_bytes(bins);
     int wpb = kMaxWavesPerBlock;
     while (wpb > 1 && thr_bytes + (size_t)A.hist_stride * wpb > kLdsBytes) wpb /= 2;
     const size_t lds = thr_bytes + (size_t)A.hist_stride * wpb;
@@ -1078,7 +1134,15 @@ void count_fast(Ctx &ctx, std::vector<FastSample> &samples, const mia_bins &bins
     if (nitems < waves) waves = (int)std::max<int64_t>(1, nitems);
     A.n_waves = waves;
     unsigned grid = (unsigned)((waves + wpb - 1) / wpb);
-    if (nitems > 0) {
+    for (int pass = 0; pass < npass && nitems > 0; ++pass) {
+        const int r0 = pass * nrp, r1 = std::min(n_r, r0 + nrp);
+        A.n_r = r1 - r0;
+        A.nbins = (r1 - r0) * n_2;
+        A.bin_off = r0 * n_2;
+        A.T = T + r0;
+        A.T0 = bins.r2_edges[r0];
+        A.Tn = bins.r2_edges[r1];
+        A.lgc = (float)(-l0 * lgs - r0);
         if (bins.mode == MIA_RPPI)
             launch_pairs_mode<MIA_RPPI>(A, grid, wpb, lds, st);
         else
