@@ -135,9 +135,10 @@ typedef struct mia_tuning {
     int32_t hist_copies;        /* interleaved LDS copies of each histogram bin (1,2,4) */
     int32_t cell_shapes;        /* > 0: size cells to hold this many shapes on average
                                    instead of the cost-model search                     */
-    int32_t r_bins_per_pass;    /* > 0: r bins per pass of the pair kernel (a binning
-                                   whose wave histograms would leave fewer than 8 waves
-                                   per CU is counted in passes over r-bin ranges)      */
+    int32_t r_bins_per_pass;    /* > 0: r bins per pass of the pair kernel, clamped to
+                                   what one wave's LDS histogram holds (large binnings
+                                   are counted in passes over r-bin ranges; by default
+                                   the library picks the pass size)                    */
 } mia_tuning;
 
 /* Per-call statistics, filled when the pointer is non-NULL. */

@@ -420,12 +420,12 @@ __device__ void flush_hist(const PairArgs &P, char *hist, const SampleDev &S, in
         if (dd && S.dd) atomicAdd(&S.dd[gb], vdd);
         if (vr && S.var) atomicAdd(&S.var[gb], vvr);
         if (S.K > 0) {
-            int o = (1 + rs) * BO + gb;
+            int64_t o = (int64_t)(1 + rs) * BO + gb;
             atomicAdd(&S.npairs[o], (unsigned long long)c);
             atomicAdd(&S.sp[o], vsp);
             if (dd && S.dd) atomicAdd(&S.dd[o], vdd);
             if (rp != rs) {
-                o = (1 + rp) * BO + gb;
+                o = (int64_t)(1 + rp) * BO + gb;
                 atomicAdd(&S.npairs[o], (unsigned long long)c);
                 atomicAdd(&S.sp[o], vsp);
                 if (dd && S.dd) atomicAdd(&S.dd[o], vdd);

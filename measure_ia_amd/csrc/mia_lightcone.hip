@@ -610,6 +610,7 @@ struct LcPairArgs {
     double cut, cut_lo, cut_hi;  // (r, mu): r_p^2 threshold of rp_cut and its guard band
     float lgs, lgc, xs, xc, dr, d2;
     int scan;                    // 1: the exact path scans every threshold (guard band of a whole bin)
+    int bin_off, nbins_out;      // this pass's first output bin; bins per output row (B)
     int hist_stride, off_dd, off_sp, off_sx;
     unsigned long long *npairs;  // (1+P)*B
     double *dd, *sp, *sx;        // (1+P)*B (sp, sx NULL for pair counts)
@@ -641,7 +642,7 @@ __device__ void lc_flush(const LcPairArgs &P, char *hist, int pn, int ps, int la
     double *hdd = (double *)(hist + P.off_dd);
     double *hsp = (double *)(hist + P.off_sp);
     double *hsx = (double *)(hist + P.off_sx);
-    const int B = P.nbins;
+    const int B = P.nbins, BO = P.nbins_out;
     for (int b = lane; b < B; b += kWave) {
         uint32_t c = cnt[b];
         if (c == 0) continue;
@@ -659,7 +660,7 @@ __device__ void lc_flush(const LcPairArgs &P, char *hist, int pn, int ps, int la
         }
         for (int k = 0; k < 3; ++k) {
             if (tgt[k] < 0) continue;
-            int o = tgt[k] * B + b;
+            const int64_t o = (int64_t)tgt[k] * BO + P.bin_off + b;
             atomicAdd(&P.npairs[o], (unsigned long long)c);
             atomicAdd(&P.dd[o], vdd);
             if (SHAPES) {
@@ -1003,9 +1004,11 @@ size_t lc_hist_bytes(int B, bool shapes) {
 
 }  // namespace
 
+// Larger binnings run in passes over r-bin ranges, so one r row of the
+// histogram (with every threshold) has to fit.
 bool lightcone_fits(const mia_bins &bins) {
     size_t thr = ((size_t)(bins.n_r + 1 + bins.n_2 + 1) * 8 + 15) & ~(size_t)15;
-    return thr + lc_hist_bytes(bins.n_r * bins.n_2, true) <= kLcLds;
+    return thr + lc_hist_bytes(bins.n_2, true) <= kLcLds;
 }
 
 void count_lightcone(Ctx &ctx, const mia_lc_sample &s, const mia_bins &bins, const mia_counts &out,
@@ -1311,9 +1314,8 @@ void count_lightcone(Ctx &ctx, const mia_lc_sample &s, const mia_bins &bins, con
     A.sc2w = sc2w;
     A.ss2w = ss2w;
     A.sw = sw;
-    A.n_r = n_r;
     A.n_2 = n_2;
-    A.nbins = B;
+    A.nbins_out = B;
     A.P = P;
     double *Td = ctx.arena.get<double>("lc/T", n_r + 1);
     double *Xd = ctx.arena.get<double>("lc/X", n_2 + 1);
@@ -1322,10 +1324,7 @@ void count_lightcone(Ctx &ctx, const mia_lc_sample &s, const mia_bins &bins, con
     for (int i = 0; i < nx2; ++i) xh[i] = bins.x2_edges[i];
     MIA_HIP(hipMemcpyAsync(Td, bins.r2_edges, sizeof(double) * (n_r + 1), hipMemcpyHostToDevice, st));
     MIA_HIP(hipMemcpyAsync(Xd, xh.data(), sizeof(double) * (n_2 + 1), hipMemcpyHostToDevice, st));
-    A.T = Td;
     A.X = Xd;
-    A.T0lo = bins.r2_edges[0] * (1.0 - kDomainSlack);
-    A.Tnhi = bins.r2_edges[n_r] * (1.0 + kDomainSlack);
     A.h = s.proj_scale;
     A.cut = s.rp2_cut;
     A.cut_lo = s.rp2_cut > 0 ? s.rp2_cut * (1.0 - kDomainSlack) : s.rp2_cut;
@@ -1335,8 +1334,8 @@ void count_lightcone(Ctx &ctx, const mia_lc_sample &s, const mia_bins &bins, con
     double l0 = std::log2(bins.r2_edges[0]), ln = std::log2(bins.r2_edges[n_r]);
     double lgs = n_r / (ln - l0);
     A.lgs = (float)lgs;
-    A.lgc = (float)(-l0 * lgs);
-    double err_r = (4 * u * std::fmax(std::fabs(l0), std::fabs(ln)) + 4 * u) * lgs + 2 * u * n_r;
+    // (4u n_r: a pass's constant -l0 lgs - r0 is rounded to FP32 once more)
+    double err_r = (4 * u * std::fmax(std::fabs(l0), std::fabs(ln)) + 4 * u) * lgs + 4 * u * n_r;
     double err_2;
     if (bins.mode == MIA_RPPI) {
         A.X0 = bins.x2_edges[0];
@@ -1358,29 +1357,78 @@ void count_lightcone(Ctx &ctx, const mia_lc_sample &s, const mia_bins &bins, con
     A.dr = exact_only ? 1.0f : (float)dr;
     A.d2 = exact_only ? 1.0f : (float)d2;
     A.scan = exact_only ? 1 : 0;
-    int off = ((4 * B) + 15) & ~15;
+    // Passes over r-bin ranges, as in the box counter: pass [r0, r1) bins the
+    // pairs with T[r0] <= r^2 < T[r1] (the exact path decides at the pass's
+    // ends, so every pair lands in one pass) into output bins r0 * n_2 + local.
+    // The pass size minimises passes / min(1, w / 16), w the waves per CU
+    // that the pass's histograms leave room for, with at least 4 waves per
+    // CU where any split allows it (ties: fewer passes).
+    const size_t thr_bytes = ((size_t)(n_r + 1 + n_2 + 1) * 8 + 15) & ~(size_t)15;
+    auto wpb_for = [&](int nr_) {
+        int w = kLcWavesPerBlock;
+        while (w > 1 && thr_bytes + lc_hist_bytes(nr_ * n_2, shapes) * w > kLcLds) w /= 2;
+        return w;
+    };
+    auto waves_per_cu = [&](int nr_) {
+        const int w = wpb_for(nr_);
+        const size_t l = thr_bytes + lc_hist_bytes(nr_ * n_2, shapes) * w;
+        if (l > kLcLds) return 0;
+        return (int)std::min<size_t>(32 / w, kLcLds / l) * w;
+    };
+    int nrp = n_r;
+    if (tuning && tuning->r_bins_per_pass > 0) {
+        nrp = std::min(n_r, tuning->r_bins_per_pass);
+        while (nrp > 1 && waves_per_cu(nrp) == 0) --nrp;
+    } else {
+        double best = 1e300;
+        int best_p = 0;
+        for (int pass_min_waves : {4, 1}) {
+            for (int c = n_r; c >= 1; --c) {
+                const int w = waves_per_cu(c);
+                if (w < pass_min_waves) continue;
+                const int passes = (n_r + c - 1) / c;
+                const double cost = (double)passes / std::min(1.0, w / 16.0);
+                if (cost < best * (1 - 1e-9) || (cost <= best * (1 + 1e-9) && passes < best_p)) {
+                    best = cost;
+                    best_p = passes;
+                    nrp = c;
+                }
+            }
+            if (best_p) break;
+        }
+    }
+    const int npass = (n_r + nrp - 1) / nrp;
+    const int Bp = nrp * n_2;  // bins of the largest pass
+    int off = ((4 * Bp) + 15) & ~15;
     A.off_dd = off;
-    off += 8 * B;
+    off += 8 * Bp;
     A.off_sp = off;
-    if (shapes) off += 8 * B;
+    if (shapes) off += 8 * Bp;
     A.off_sx = off;
-    if (shapes) off += 8 * B;
+    if (shapes) off += 8 * Bp;
     A.hist_stride = (off + 15) & ~15;
     A.npairs = (unsigned long long *)out.npairs;
     A.dd = out.dd;
     A.sp = out.splus;
     A.sx = out.scross;
     A.rechecks = counters;
-    const size_t thr_bytes = ((size_t)(n_r + 1 + n_2 + 1) * 8 + 15) & ~(size_t)15;
-    int wpb = kLcWavesPerBlock;
-    while (wpb > 1 && thr_bytes + (size_t)A.hist_stride * wpb > kLcLds) wpb /= 2;
+    const int wpb = wpb_for(nrp);
     const size_t lds = thr_bytes + (size_t)A.hist_stride * wpb;
+    MIA_REQUIRE(lds <= kLcLds, "binning too large for the lightcone kernel's LDS histogram");
     int blocks_per_cu = (int)std::max<size_t>(1, std::min<size_t>(32 / wpb, kLcLds / lds));
     int waves = tuning && tuning->waves > 0 ? tuning->waves : ctx.cu_count * blocks_per_cu * wpb;
     if (nitems < waves) waves = std::max(1, nitems);
     A.n_waves = waves;
     unsigned grid = (unsigned)((waves + wpb - 1) / wpb);
-    if (nitems > 0) {
+    for (int pass = 0; pass < npass && nitems > 0; ++pass) {
+        const int r0 = pass * nrp, r1 = std::min(n_r, r0 + nrp);
+        A.n_r = r1 - r0;
+        A.nbins = (r1 - r0) * n_2;
+        A.bin_off = r0 * n_2;
+        A.T = Td + r0;
+        A.T0lo = bins.r2_edges[r0] * (1.0 - kDomainSlack);
+        A.Tnhi = bins.r2_edges[r1] * (1.0 + kDomainSlack);
+        A.lgc = (float)(-l0 * lgs - r0);
         if (bins.mode == MIA_RPPI) {
             if (shapes)
                 lc_launch<MIA_RPPI, true>(A, grid, wpb, lds, st);

@@ -117,3 +117,27 @@ def test_fullsize_multipoles_leave_one_patch_out(engine):
                                                      ra_s=ra[keep], dec_s=dec[keep], chi_s=chi[keep], rp2_cut=cut),
                                      th)
         np.testing.assert_array_equal(sub.npairs[0], full.npairs[0] - full.npairs[1 + i])
+
+
+@pytest.mark.parametrize("mode, nr, n2", [("rppi", 60, 60), ("rmu", 100, 100)])
+def test_lightcone_large_binning_in_passes(engine, mode, nr, n2):
+    """3,600 and 10,000 (r, x) bins with shapes and patches: one wave's
+    histogram of the whole binning no longer leaves room for enough waves, so
+    the lightcone kernel counts in passes over r-bin ranges.  Counts equal the
+    oracle, and explicit pass sizes of 1, 7 and all r bins give the same."""
+    a, cfg = make_case("rppi_shapes_patches" if mode == "rppi" else "rmu_shapes_patches")
+    cfg.update(nr=nr, n2=n2)
+    _, _, th = bins(cfg)
+    print(f"large lightcone binning {mode} {nr}x{n2}", flush=True)
+    lc = device_arrays(a, cfg)
+    res = engine.count_lightcone(lc, th)
+    ref = oracle_counts(a, cfg)
+    assert ref["NP"][0].sum() > 1000
+    np.testing.assert_array_equal(res.npairs, ref["NP"])
+    assert np.abs(res.dd - ref["DD"]).max() <= 1e-12 * np.abs(ref["DD"]).max()
+    assert _rel(res.splus, ref["SplusD"], ref["SplusD_abs"]).max() <= 1e-6
+    assert _rel(res.scross, ref["ScrossD"], ref["ScrossD_abs"]).max() <= 1e-6
+    for per_pass in (1, 7, nr):
+        r = engine.count_lightcone(lc, th, tuning={"r_bins_per_pass": per_pass})
+        np.testing.assert_array_equal(r.npairs, res.npairs)
+        assert np.abs(r.splus - res.splus).max() <= 1e-12 * np.abs(res.splus).max()
