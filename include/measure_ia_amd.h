@@ -131,7 +131,7 @@ typedef struct mia_tuning {
     int32_t slabs_per_block;    /* shape-block height in slabs                        */
     int32_t block_shapes;       /* shapes per block, <= 64                            */
     int32_t item_positions;     /* max positions per work item                        */
-    int32_t waves;              /* persistent waves of the pair kernel                */
+    int32_t waves;              /* persistent waves of the pair kernel (at most 2^20)  */
     int32_t hist_copies;        /* interleaved LDS copies of each histogram bin (1,2,4) */
     int32_t cell_shapes;        /* > 0: size cells to hold this many shapes on average
                                    instead of the cost-model search                     */

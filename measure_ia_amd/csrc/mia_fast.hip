@@ -354,6 +354,8 @@ struct PairArgs {
     int64_t n_items;
     const int64_t *total_cost;   // device: cost prefix total (read by the kernel)
     int n_waves;
+    const int64_t *bounds;       // schedule unit -> first item, n_waves * kScheduleUnits + 2
+    int *queue;                  // tail-chunk counter, zero at launch
     int mode, n_r, n_2, weighted, var, nbins;
     const double *T, *X;         // exact thresholds (global copy)
     double T0, Tn, X0, Xn;       // domain bounds (rppi: s and pi; rmu: r^2)
@@ -703,57 +705,45 @@ __global__ __launch_bounds__(kThreads) void k_pairs(PairArgs P) {
     // position cells) stay on one XCD
     const int w = xcd_run_block() * (blockDim.x / kWave) + wv;
     if (w >= P.n_waves || P.n_items == 0) return;
-    const MIA_CONST int64_t *cpre = (const MIA_CONST int64_t *)P.cost_prefix;
-    const double total_cost = (double)*(const MIA_CONST int64_t *)P.total_cost;
-    // items [it0, it1) whose cost prefix falls in this wave's share
-    auto first_at_least = [&](int64_t target) {
-        int64_t lo = 0, hi = P.n_items;
-        while (lo < hi) {
-            int64_t mid = (lo + hi) >> 1;
-            // prefix[mid] * W >= target * C (any monotone rule works: the
-            // neighbouring wave evaluates the same boundary)
-            if ((double)cpre[mid] * P.n_waves >= (double)target * total_cost)
-                hi = mid;
-            else
-                lo = mid + 1;
-        }
-        return lo;
-    };
-    const int64_t it0 = first_at_least(w);
-    const int64_t it1 = first_at_least(w + 1);
+    // a contiguous share of the cost-sorted items, then tail chunks from the counter
+    const WaveSchedule sch = make_schedule(P.bounds, P.n_waves);
+    int64_t it0, it1;
+    sch.first(w, it0, it1);
     const MIA_CONST int32_t *items = (const MIA_CONST int32_t *)P.items;
     const MIA_CONST int32_t *blocks = (const MIA_CONST int32_t *)P.blocks;
     int cur_key_s = -1, cur_rs = -1, cur_rp = -1;
     const SampleDev *Sp = nullptr;
     SampleView S{};
     uint32_t nre = 0;
-    for (int64_t i = it0; i < it1; ++i) {
-        const MIA_CONST int32_t *ip = items + 4 * i;
-        const int blk = ip[0], p0 = ip[1], np = ip[2], w3 = ip[3];
-        const int rp = (int16_t)(w3 & 0xffff);
-        const int shifts = (w3 >> 16) & 0xff;
-        const MIA_CONST int32_t *bp = blocks + 6 * (int64_t)blk;
-        const int s0 = bp[0], ns = bp[1], sample = bp[4], rs = bp[5];
-        if (sample != cur_key_s || rs != cur_rs || rp != cur_rp) {
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            if (Sp) flush_hist(P, hist, *Sp, cur_rs, cur_rp, lane);
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            if (sample != cur_key_s) {
-                Sp = &P.samples[sample];
-                S = load_view(Sp);
+    do {
+        for (int64_t i = it0; i < it1; ++i) {
+            const MIA_CONST int32_t *ip = items + 4 * i;
+            const int blk = ip[0], p0 = ip[1], np = ip[2], w3 = ip[3];
+            const int rp = (int16_t)(w3 & 0xffff);
+            const int shifts = (w3 >> 16) & 0xff;
+            const MIA_CONST int32_t *bp = blocks + 6 * (int64_t)blk;
+            const int s0 = bp[0], ns = bp[1], sample = bp[4], rs = bp[5];
+            if (sample != cur_key_s || rs != cur_rs || rp != cur_rp) {
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                if (Sp) flush_hist(P, hist, *Sp, cur_rs, cur_rp, lane);
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                if (sample != cur_key_s) {
+                    Sp = &P.samples[sample];
+                    S = load_view(Sp);
+                }
+                cur_key_s = sample;
+                cur_rs = rs;
+                cur_rp = rp;
             }
-            cur_key_s = sample;
-            cur_rs = rs;
-            cur_rp = rp;
+            ShapePtr shp = (ShapePtr)(S.srec + 8 * (int64_t)s0);
+            if (shifts & 64)
+                run_item<MODE, SH_WRAP, WEIGHTED, VAR>(P, S, p0, np, shifts, shp, ns, hist, T, X, lane, nre);
+            else if (shifts & 63)
+                run_item<MODE, SH_CONST, WEIGHTED, VAR>(P, S, p0, np, shifts, shp, ns, hist, T, X, lane, nre);
+            else
+                run_item<MODE, SH_NONE, WEIGHTED, VAR>(P, S, p0, np, shifts, shp, ns, hist, T, X, lane, nre);
         }
-        ShapePtr shp = (ShapePtr)(S.srec + 8 * (int64_t)s0);
-        if (shifts & 64)
-            run_item<MODE, SH_WRAP, WEIGHTED, VAR>(P, S, p0, np, shifts, shp, ns, hist, T, X, lane, nre);
-        else if (shifts & 63)
-            run_item<MODE, SH_CONST, WEIGHTED, VAR>(P, S, p0, np, shifts, shp, ns, hist, T, X, lane, nre);
-        else
-            run_item<MODE, SH_NONE, WEIGHTED, VAR>(P, S, p0, np, shifts, shp, ns, hist, T, X, lane, nre);
-    }
+    } while (sch.next(P.queue, lane, it0, it1));
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     if (Sp) flush_hist(P, hist, *Sp, cur_rs, cur_rp, lane);
     if (nre) atomicAdd(P.rechecks, (unsigned long long)nre);
@@ -946,6 +936,9 @@ void count_fast(Ctx &ctx, std::vector<FastSample> &samples, const mia_bins &bins
     unsigned long long *counters = ctx.arena.get<unsigned long long>("fast/counters", 2);
     fill<unsigned long long>(counters, 0ull, 2, st);
     unsigned long long *ncand = counters + 1;
+    // one tail-chunk counter per r-bin pass (n_r <= kMaxBinsR)
+    int32_t *queue = ctx.arena.get<int32_t>("fast/queue", kMaxBinsR);
+    fill<int32_t>(queue, 0, kMaxBinsR, st);
     for (int si = 0; si < ns_; ++si) {
         PlanArgs P = plans[si];
         P.emit = 1;
@@ -1130,10 +1123,13 @@ void count_fast(Ctx &ctx, std::vector<FastSample> &samples, const mia_bins &bins
         throw Error{MIA_E_UNSUPPORTED, "histogram too large for LDS (n_r*n_2 too big for MIA_FAST)"};
     // at most 8 waves per SIMD (32 per CU)
     int blocks_per_cu = (int)std::max<size_t>(1, std::min<size_t>(32 / wpb, kLdsBytes / lds));
-    int waves = tuning && tuning->waves > 0 ? tuning->waves : ctx.cu_count * blocks_per_cu * wpb;
+    int waves = tuning && tuning->waves > 0 ? std::min(tuning->waves, kMaxWaves) : ctx.cu_count * blocks_per_cu * wpb;
     if (nitems < waves) waves = (int)std::max<int64_t>(1, nitems);
     A.n_waves = waves;
     unsigned grid = (unsigned)((waves + wpb - 1) / wpb);
+    int64_t *bounds = ctx.arena.get<int64_t>("fast/bounds", (size_t)waves * kScheduleUnits + 2);
+    if (nitems > 0) schedule_bounds(cpre, nitems, waves, bounds, st);
+    A.bounds = bounds;
     for (int pass = 0; pass < npass && nitems > 0; ++pass) {
         const int r0 = pass * nrp, r1 = std::min(n_r, r0 + nrp);
         A.n_r = r1 - r0;
@@ -1143,6 +1139,7 @@ void count_fast(Ctx &ctx, std::vector<FastSample> &samples, const mia_bins &bins
         A.T0 = bins.r2_edges[r0];
         A.Tn = bins.r2_edges[r1];
         A.lgc = (float)(-l0 * lgs - r0);
+        A.queue = queue + pass;
         if (bins.mode == MIA_RPPI)
             launch_pairs_mode<MIA_RPPI>(A, grid, wpb, lds, st);
         else

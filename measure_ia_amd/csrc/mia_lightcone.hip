@@ -600,6 +600,8 @@ struct LcPairArgs {
     const int64_t *total_cost;
     int64_t n_items;
     int n_waves;
+    const int64_t *bounds;       // schedule unit -> first item, n_waves * kScheduleUnits + 2
+    int *queue;                  // tail-chunk counter per pass, zero at launch
     const double *prec;          // position records
     const double *sra, *sdec, *schi, *sc2w, *ss2w, *sw;
     int n_r, n_2, nbins, P;      // P: patches (0: no tallies)
@@ -792,22 +794,10 @@ __global__ __launch_bounds__(kLcThreads) void k_lc_pairs(LcPairArgs P) {
     // same position blocks) stay on one XCD
     const int w = xcd_run_block() * (blockDim.x / kWave) + wv;
     if (w >= P.n_waves || P.n_items == 0) return;
-    const __attribute__((address_space(4))) int64_t *cpre =
-        (const __attribute__((address_space(4))) int64_t *)P.cost_prefix;
-    const double total_cost = (double)*(const __attribute__((address_space(4))) int64_t *)P.total_cost;
-    auto first_at_least = [&](int64_t target) {
-        int64_t lo = 0, hi = P.n_items;
-        while (lo < hi) {
-            int64_t mid = (lo + hi) >> 1;
-            if ((double)cpre[mid] * P.n_waves >= (double)target * total_cost)
-                hi = mid;
-            else
-                lo = mid + 1;
-        }
-        return lo;
-    };
-    const int64_t it0 = first_at_least(w);
-    const int64_t it1 = first_at_least(w + 1);
+    // a contiguous share of the patch-sorted items, then tail chunks from the counter
+    const WaveSchedule sch = make_schedule(P.bounds, P.n_waves);
+    int64_t it0, it1;
+    sch.first(w, it0, it1);
     CPtrI items = (CPtrI)P.items;
     CPtrI blocks = (CPtrI)P.blocks;
     CPtrI ranges = (CPtrI)P.ranges;
@@ -818,108 +808,110 @@ __global__ __launch_bounds__(kLcThreads) void k_lc_pairs(LcPairArgs P) {
     float vlgc, vxc;
     asm volatile("v_mov_b32 %0, %1" : "=v"(vlgc) : "s"(P.lgc));
     asm volatile("v_mov_b32 %0, %1" : "=v"(vxc) : "s"(P.xc));
-    for (int64_t i = it0; i < it1; ++i) {
-        CPtrI ip = items + 4 * i;
-        const int blk = ip[0], r0 = ip[1], nr = ip[2], ns = ip[3];
-        CPtrI bp = blocks + 8 * (int64_t)blk;
-        const int p0 = bp[0], np = bp[1], pn = bp[3];
-        const int ps = ranges[4 * (int64_t)r0 + 2];
-        if (pn != cur_pn || ps != cur_ps) {
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            if (cur_pn >= 0) lc_flush<SHAPES>(P, hist, cur_pn, cur_ps, lane);
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            cur_pn = pn;
-            cur_ps = ps;
-        }
-        LcPend pd{-1, 0, 0.0, 0.0, 0.0};
-        int j = r0, o = 0;
-        const int jend = r0 + nr;
-        for (int done = 0; done < ns; done += kWave) {
-            // fill the 64 lanes from consecutive ranges (wave-uniform walk)
-            int my = -1, filled = 0;
-            while (filled < kWave && j < jend) {
-                const int rs0 = ranges[4 * (int64_t)j], rn = ranges[4 * (int64_t)j + 1];
-                const int take = rn - o < kWave - filled ? rn - o : kWave - filled;
-                if (lane >= filled && lane < filled + take) my = rs0 + o + (lane - filled);
-                filled += take;
-                o += take;
-                if (o >= rn) {
-                    ++j;
-                    o = 0;
-                }
+    do {
+        for (int64_t i = it0; i < it1; ++i) {
+            CPtrI ip = items + 4 * i;
+            const int blk = ip[0], r0 = ip[1], nr = ip[2], ns = ip[3];
+            CPtrI bp = blocks + 8 * (int64_t)blk;
+            const int p0 = bp[0], np = bp[1], pn = bp[3];
+            const int ps = ranges[4 * (int64_t)r0 + 2];
+            if (pn != cur_pn || ps != cur_ps) {
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                if (cur_pn >= 0) lc_flush<SHAPES>(P, hist, cur_pn, cur_ps, lane);
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                cur_pn = pn;
+                cur_ps = ps;
             }
-            if (filled <= kWave / 2 && np > 1) {
-                // Few shapes: R replicas of the chunk's shapes fill the
-                // lanes, replica r taking positions r, r + R, ... of the
-                // block, whose records are then read per lane.
-                const int R = kWave / filled < np ? kWave / filled : np;
-                const int rep = lane / filled;
-                const int msh = __shfl(my, lane - rep * filled);
-                const bool sv = (rep < R) & (msh >= 0);
-                const int q = sv ? msh : 0;
+            LcPend pd{-1, 0, 0.0, 0.0, 0.0};
+            int j = r0, o = 0;
+            const int jend = r0 + nr;
+            for (int done = 0; done < ns; done += kWave) {
+                // fill the 64 lanes from consecutive ranges (wave-uniform walk)
+                int my = -1, filled = 0;
+                while (filled < kWave && j < jend) {
+                    const int rs0 = ranges[4 * (int64_t)j], rn = ranges[4 * (int64_t)j + 1];
+                    const int take = rn - o < kWave - filled ? rn - o : kWave - filled;
+                    if (lane >= filled && lane < filled + take) my = rs0 + o + (lane - filled);
+                    filled += take;
+                    o += take;
+                    if (o >= rn) {
+                        ++j;
+                        o = 0;
+                    }
+                }
+                if (filled <= kWave / 2 && np > 1) {
+                    // Few shapes: R replicas of the chunk's shapes fill the
+                    // lanes, replica r taking positions r, r + R, ... of the
+                    // block, whose records are then read per lane.
+                    const int R = kWave / filled < np ? kWave / filled : np;
+                    const int rep = lane / filled;
+                    const int msh = __shfl(my, lane - rep * filled);
+                    const bool sv = (rep < R) & (msh >= 0);
+                    const int q = sv ? msh : 0;
+                    const double qra = P.sra[q], qdec = P.sdec[q], qchi = P.schi[q], qw = P.sw[q];
+                    const double qc2 = SHAPES ? P.sc2w[q] : 0.0, qs2 = SHAPES ? P.ss2w[q] : 0.0;
+                    int k = rep;
+                    bool v = sv & (k < np);
+                    const double *rec = P.prec + 8 * ((int64_t)p0 + (v ? k : 0));
+                    double a0 = rec[0], a1 = rec[1], a2 = rec[2], a3 = rec[3], a4 = rec[4], a5 = rec[5],
+                           a6 = rec[6];
+                    const int nt = (np + R - 1) / R;
+                    for (int t = 0; t < nt; ++t) {
+                        const int kn = k + R;
+                        const bool vn = sv & (kn < np);
+                        const double *rn = P.prec + 8 * ((int64_t)p0 + (vn ? kn : 0));
+                        const double b0 = rn[0], b1 = rn[1], b2 = rn[2], b3 = rn[3], b4 = rn[4], b5 = rn[5],
+                                     b6 = rn[6];
+                        lc_pair<MODE, SHAPES>(P, v, qra, qdec, qchi, qc2, qs2, qw, a0, a1, a2, a3, a4, a5, a6,
+                                              hist, T, X, nre, pd, vlgc, vxc);
+                        a0 = b0;
+                        a1 = b1;
+                        a2 = b2;
+                        a3 = b3;
+                        a4 = b4;
+                        a5 = b5;
+                        a6 = b6;
+                        v = vn;
+                        k = kn;
+                    }
+                    continue;
+                }
+                const bool valid = my >= 0;
+                const int q = valid ? my : 0;
                 const double qra = P.sra[q], qdec = P.sdec[q], qchi = P.schi[q], qw = P.sw[q];
                 const double qc2 = SHAPES ? P.sc2w[q] : 0.0, qs2 = SHAPES ? P.ss2w[q] : 0.0;
-                int k = rep;
-                bool v = sv & (k < np);
-                const double *rec = P.prec + 8 * ((int64_t)p0 + (v ? k : 0));
-                double a0 = rec[0], a1 = rec[1], a2 = rec[2], a3 = rec[3], a4 = rec[4], a5 = rec[5],
-                       a6 = rec[6];
-                const int nt = (np + R - 1) / R;
-                for (int t = 0; t < nt; ++t) {
-                    const int kn = k + R;
-                    const bool vn = sv & (kn < np);
-                    const double *rn = P.prec + 8 * ((int64_t)p0 + (vn ? kn : 0));
-                    const double b0 = rn[0], b1 = rn[1], b2 = rn[2], b3 = rn[3], b4 = rn[4], b5 = rn[5],
-                                 b6 = rn[6];
-                    lc_pair<MODE, SHAPES>(P, v, qra, qdec, qchi, qc2, qs2, qw, a0, a1, a2, a3, a4, a5, a6,
-                                          hist, T, X, nre, pd, vlgc, vxc);
-                    a0 = b0;
-                    a1 = b1;
-                    a2 = b2;
-                    a3 = b3;
-                    a4 = b4;
-                    a5 = b5;
-                    a6 = b6;
-                    v = vn;
-                    k = kn;
+                // two positions per iteration, the record after them loaded ahead
+                // (scalar loads): 10 % faster than one position with a one-ahead
+                // prefetch, whose record copies cost scalar instructions each pass
+                CPtrD pr = (CPtrD)(P.prec + 8 * (int64_t)p0);
+                double a0 = pr[0], a1 = pr[1], a2 = pr[2], a3 = pr[3], a4 = pr[4], a5 = pr[5], a6 = pr[6];
+                int k = 0;
+                for (; k + 1 < np; k += 2) {
+                    CPtrD r1 = pr + 8 * (k + 1);
+                    const double b0 = r1[0], b1 = r1[1], b2 = r1[2], b3 = r1[3], b4 = r1[4], b5 = r1[5],
+                                 b6 = r1[6];
+                    lc_pair<MODE, SHAPES>(P, valid, qra, qdec, qchi, qc2, qs2, qw, a0, a1, a2, a3, a4, a5,
+                                          a6, hist, T, X, nre, pd, vlgc, vxc);
+                    CPtrD r2 = pr + 8 * (k + 2 < np ? k + 2 : k + 1);
+                    const double c0 = r2[0], c1 = r2[1], c2 = r2[2], c3 = r2[3], c4 = r2[4], c5 = r2[5],
+                                 c6 = r2[6];
+                    lc_pair<MODE, SHAPES>(P, valid, qra, qdec, qchi, qc2, qs2, qw, b0, b1, b2, b3, b4, b5,
+                                          b6, hist, T, X, nre, pd, vlgc, vxc);
+                    a0 = c0;
+                    a1 = c1;
+                    a2 = c2;
+                    a3 = c3;
+                    a4 = c4;
+                    a5 = c5;
+                    a6 = c6;
                 }
-                continue;
+                if (k < np)
+                    lc_pair<MODE, SHAPES>(P, valid, qra, qdec, qchi, qc2, qs2, qw, a0, a1, a2, a3, a4, a5,
+                                          a6, hist, T, X, nre, pd, vlgc, vxc);
             }
-            const bool valid = my >= 0;
-            const int q = valid ? my : 0;
-            const double qra = P.sra[q], qdec = P.sdec[q], qchi = P.schi[q], qw = P.sw[q];
-            const double qc2 = SHAPES ? P.sc2w[q] : 0.0, qs2 = SHAPES ? P.ss2w[q] : 0.0;
-            // two positions per iteration, the record after them loaded ahead
-            // (scalar loads): 10 % faster than one position with a one-ahead
-            // prefetch, whose record copies cost scalar instructions each pass
-            CPtrD pr = (CPtrD)(P.prec + 8 * (int64_t)p0);
-            double a0 = pr[0], a1 = pr[1], a2 = pr[2], a3 = pr[3], a4 = pr[4], a5 = pr[5], a6 = pr[6];
-            int k = 0;
-            for (; k + 1 < np; k += 2) {
-                CPtrD r1 = pr + 8 * (k + 1);
-                const double b0 = r1[0], b1 = r1[1], b2 = r1[2], b3 = r1[3], b4 = r1[4], b5 = r1[5],
-                             b6 = r1[6];
-                lc_pair<MODE, SHAPES>(P, valid, qra, qdec, qchi, qc2, qs2, qw, a0, a1, a2, a3, a4, a5,
-                                      a6, hist, T, X, nre, pd, vlgc, vxc);
-                CPtrD r2 = pr + 8 * (k + 2 < np ? k + 2 : k + 1);
-                const double c0 = r2[0], c1 = r2[1], c2 = r2[2], c3 = r2[3], c4 = r2[4], c5 = r2[5],
-                             c6 = r2[6];
-                lc_pair<MODE, SHAPES>(P, valid, qra, qdec, qchi, qc2, qs2, qw, b0, b1, b2, b3, b4, b5,
-                                      b6, hist, T, X, nre, pd, vlgc, vxc);
-                a0 = c0;
-                a1 = c1;
-                a2 = c2;
-                a3 = c3;
-                a4 = c4;
-                a5 = c5;
-                a6 = c6;
-            }
-            if (k < np)
-                lc_pair<MODE, SHAPES>(P, valid, qra, qdec, qchi, qc2, qs2, qw, a0, a1, a2, a3, a4, a5,
-                                      a6, hist, T, X, nre, pd, vlgc, vxc);
+            if (pd.b >= 0) lc_hist_add<SHAPES>(P, hist, pd);
         }
-        if (pd.b >= 0) lc_hist_add<SHAPES>(P, hist, pd);
-    }
+    } while (sch.next(P.queue, lane, it0, it1));
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     if (cur_pn >= 0) lc_flush<SHAPES>(P, hist, cur_pn, cur_ps, lane);
     if (nre) atomicAdd(P.rechecks, (unsigned long long)nre);
@@ -1284,6 +1276,9 @@ void count_lightcone(Ctx &ctx, const mia_lc_sample &s, const mia_bins &bins, con
     int64_t *cpre = ctx.arena.get<int64_t>("lc/cpre", nitems + 1);
     unsigned long long *counters = ctx.arena.get<unsigned long long>("lc/counters", 2);
     fill<unsigned long long>(counters, 0ull, 2, st);
+    // one tail-chunk counter per r-bin pass (n_r <= kMaxBinsR)
+    int32_t *queue = ctx.arena.get<int32_t>("lc/queue", kMaxBinsR);
+    fill<int32_t>(queue, 0, kMaxBinsR, st);
     hipLaunchKernelGGL(k_lc_items, dim3(nblk_(nblocks, 256)), dim3(256), 0, st, ranges, roff, soff,
                        nblocks, item_shapes, nit, ioff, items0, blocks, P1, ikey, cost0, counters + 1);
     MIA_HIP(hipGetLastError());
@@ -1416,10 +1411,13 @@ void count_lightcone(Ctx &ctx, const mia_lc_sample &s, const mia_bins &bins, con
     const size_t lds = thr_bytes + (size_t)A.hist_stride * wpb;
     MIA_REQUIRE(lds <= kLcLds, "binning too large for the lightcone kernel's LDS histogram");
     int blocks_per_cu = (int)std::max<size_t>(1, std::min<size_t>(32 / wpb, kLcLds / lds));
-    int waves = tuning && tuning->waves > 0 ? tuning->waves : ctx.cu_count * blocks_per_cu * wpb;
+    int waves = tuning && tuning->waves > 0 ? std::min(tuning->waves, kMaxWaves) : ctx.cu_count * blocks_per_cu * wpb;
     if (nitems < waves) waves = std::max(1, nitems);
     A.n_waves = waves;
     unsigned grid = (unsigned)((waves + wpb - 1) / wpb);
+    int64_t *bounds = ctx.arena.get<int64_t>("lc/bounds", (size_t)waves * kScheduleUnits + 2);
+    if (nitems > 0) schedule_bounds(cpre, nitems, waves, bounds, st);
+    A.bounds = bounds;
     for (int pass = 0; pass < npass && nitems > 0; ++pass) {
         const int r0 = pass * nrp, r1 = std::min(n_r, r0 + nrp);
         A.n_r = r1 - r0;
@@ -1429,6 +1427,7 @@ void count_lightcone(Ctx &ctx, const mia_lc_sample &s, const mia_bins &bins, con
         A.T0lo = bins.r2_edges[r0] * (1.0 - kDomainSlack);
         A.Tnhi = bins.r2_edges[r1] * (1.0 + kDomainSlack);
         A.lgc = (float)(-l0 * lgs - r0);
+        A.queue = queue + pass;
         if (bins.mode == MIA_RPPI) {
             if (shapes)
                 lc_launch<MIA_RPPI, true>(A, grid, wpb, lds, st);

@@ -27,6 +27,27 @@ __global__ void k_fill(T *p, T v, int64_t n) {
     for (; i < n; i += stride) p[i] = v;
 }
 
+// bounds[u] = first i in [0, n] with cost_prefix[i] * U >= u * C, for u in
+// [0, U]; bounds[U + 1] = the static units per wave.
+__global__ void k_schedule_bounds(const int64_t *cpre, int64_t n, int64_t U, int waves, int64_t *bounds) {
+    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u > U + 1) return;
+    const double C = (double)cpre[n], Ud = (double)U, t = (double)u * C;
+    if (u == U + 1) {
+        bounds[u] = C >= kTailMinCost * waves ? kStaticUnits : kScheduleUnits;
+        return;
+    }
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if ((double)cpre[mid] * Ud >= t)
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    bounds[u] = lo;
+}
+
 // Exclusive scan of one tile per block; writes the tile total to sums.
 template <class T>
 __global__ __launch_bounds__(kScanThreads) void k_scan_tile(const T *in, T *out, int64_t n,
@@ -251,6 +272,13 @@ void fill(T *p, T v, int64_t n, hipStream_t st) {
     hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)blocks), dim3(256), 0, st, p, v, n);
     MIA_HIP(hipGetLastError());
 }
+void schedule_bounds(const int64_t *cost_prefix, int64_t n, int waves, int64_t *bounds, hipStream_t st) {
+    const int64_t U = (int64_t)waves * kScheduleUnits;
+    hipLaunchKernelGGL(k_schedule_bounds, dim3((unsigned)((U + 2 + 255) / 256)), dim3(256), 0, st, cost_prefix,
+                       n, U, waves, bounds);
+    MIA_HIP(hipGetLastError());
+}
+
 template void fill<int32_t>(int32_t *, int32_t, int64_t, hipStream_t);
 template void fill<int64_t>(int64_t *, int64_t, int64_t, hipStream_t);
 template void fill<uint64_t>(uint64_t *, uint64_t, int64_t, hipStream_t);

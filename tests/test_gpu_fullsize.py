@@ -72,6 +72,14 @@ def test_uniform20k_against_tree_oracle(engine):
     assert _rel(res.splus[0] / (2 * R), ref["SplusD"], ref["SplusD_abs"]).max() <= 1e-6
     assert _rel(res.scross / (2 * R), ref["ScrossD"], ref["ScrossD_abs"]).max() <= 1e-6
     assert _rel(res.splus[1:], ref["SplusD_jk"], ref["SplusD_jk_abs"]).max() <= 1e-6
+    # Few persistent waves make every wave's share large enough that the
+    # waves also claim tail chunks from the counter: the same counts.
+    for waves in (16, 333):
+        t = engine.count(_sample(p, L, 3), th, tuning={"waves": waves})
+        assert t.stats["waves"] == waves
+        np.testing.assert_array_equal(t.npairs, res.npairs)
+        np.testing.assert_array_equal(t.dd, res.dd)
+        assert np.abs(t.splus - res.splus).max() <= 1e-12 * np.abs(res.splus).max()
 
 
 @pytest.fixture(scope="module")

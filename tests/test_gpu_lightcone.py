@@ -60,7 +60,8 @@ def test_lightcone_tuning_does_not_change_counts(engine):
     lc = device_arrays(a, cfg)
     base = engine.count_lightcone(lc, th)
     for tu in ({"slabs_per_side": 1, "columns_per_side": 1}, {"slabs_per_side": 37, "columns_per_side": 53},
-               {"block_shapes": 7, "item_positions": 33, "slabs_per_block": 1}, {"waves": 3}):
+               {"block_shapes": 7, "item_positions": 33, "slabs_per_block": 1}, {"waves": 3},
+               {"waves": 1}):
         r = engine.count_lightcone(lc, th, tuning=tu)
         np.testing.assert_array_equal(r.npairs, base.npairs)
         assert np.abs(r.splus - base.splus).max() <= 1e-12 * np.abs(base.splus).max()
