@@ -87,3 +87,8 @@ def test_random_lightcone_case(engine, case):
     if cfg["shapes"]:
         assert _rel(res.splus, ref["SplusD"], ref["SplusD_abs"]).max() <= 1e-6
         assert _rel(res.scross, ref["ScrossD"], ref["ScrossD_abs"]).max() <= 1e-6
+    # one persistent wave (which also claims the tail chunks when its share
+    # is large enough): the same counts
+    one = engine.count_lightcone(device_arrays(a, cfg), th, tuning={"waves": 1})
+    np.testing.assert_array_equal(one.npairs, res.npairs)
+    assert np.abs(one.dd - res.dd).max() <= 1e-12 * scale

@@ -64,3 +64,8 @@ def test_random_case(engine, case):
     res = engine.count(sample, thresholds(cfg, L, mode), method="fast")
     K = cfg["L_sub"] ** 3 if cfg["L_sub"] else 0
     compare(res, ref, p.R, K, cfg["weights"])
+    # one persistent wave (which also claims the tail chunks when its share
+    # is large enough): the same counts
+    one = engine.count(sample, thresholds(cfg, L, mode), method="fast", tuning={"waves": 1})
+    np.testing.assert_array_equal(one.npairs, res.npairs)
+    compare(one, ref, p.R, K, cfg["weights"])
