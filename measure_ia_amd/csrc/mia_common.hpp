@@ -41,24 +41,26 @@ __device__ __forceinline__ int xcd_run_block() {
 // prefix reaches u * C / U (schedule_bounds), so bounds[0] = 0, bounds[U] = n
 // (item costs are positive) and each item lies in exactly one unit range.
 // Wave w first takes units [w * S, (w + 1) * S) as one contiguous range; then
-// the waves claim the W * (kScheduleUnits - S) one-unit chunks of the rest, in
-// order, from a counter (zeroed before the launch), until it passes the last
-// chunk.  Waves the cost estimate under-loaded thus take work that would
-// otherwise wait on the waves it over-loaded.  S (stored at bounds[U + 1]) is
-// kStaticUnits, an eighth of the cost static (contiguous ranges keep the
-// histogram flushes few and neighbouring items on one XCD), when a wave's
-// share is at least kTailMinCost; shorter launches are all static, since
-// their claims would queue on the one counter.  Measured on the benchmark
-// workloads: pair kernels 17-25 % faster than a static split of the whole
-// cost, and no slower where the launch is short.
-constexpr int kScheduleUnits = 16;
-constexpr int kStaticUnits = 2;
-constexpr double kTailMinCost = 4.0e5;  // item-cost units per wave (about 5 ms of pair work)
-constexpr int kMaxWaves = 1 << 20;      // persistent waves per launch (tail counter stays in int)
+// the waves claim the rest in chunks of G units, in order, from a counter
+// (zeroed before the launch), until it passes the last chunk.  Waves the cost
+// estimate under-loaded thus take work that would otherwise wait on the waves
+// it over-loaded.  S and G (bounds[U + 1], bounds[U + 2]) follow a wave's
+// share of the cost, c = C / W, since every claim queues on the one counter:
+//   c >= kTailFineCost: S = 2, G = 1 (1/16 static, 30 claims per wave);
+//   c >= kTailMinCost:  S = 4, G = 2 (1/8 static, 14 claims);
+//   otherwise:          S = kScheduleUnits (all static, no claims).
+// Contiguous static ranges keep the histogram flushes few and neighbouring
+// items on one XCD.  Measured on the benchmark workloads: pair kernels
+// 20-27 % faster than a static split of the whole cost; with claims, a
+// 0.8 ms launch (lightcone1M) was 40 % slower, hence the static case.
+constexpr int kScheduleUnits = 32;
+constexpr double kTailMinCost = 4.0e5;   // item-cost units per wave (about 5 ms of pair work)
+constexpr double kTailFineCost = 8.0e5;  // (about 10 ms)
+constexpr int kMaxWaves = 1 << 20;       // persistent waves per launch (tail counter stays in int)
 
 struct WaveSchedule {
     const __attribute__((address_space(4))) int64_t *bounds;
-    int W, S, tail;
+    int W, S, G, tail;
 
     // the static range of wave w
     __device__ __forceinline__ void first(int w, int64_t &i0, int64_t &i1) const {
@@ -72,9 +74,9 @@ struct WaveSchedule {
         if (lane == 0) k = atomicAdd(counter, 1);
         k = __builtin_amdgcn_readfirstlane(k);
         if (k >= tail) return false;
-        const int64_t u = (int64_t)W * S + k;
+        const int64_t u = (int64_t)W * S + (int64_t)k * G;
         i0 = bounds[u];
-        i1 = bounds[u + 1];
+        i1 = bounds[u + G];
         return true;
     }
 };
@@ -83,12 +85,14 @@ __device__ __forceinline__ WaveSchedule make_schedule(const int64_t *bounds, int
     WaveSchedule s;
     s.bounds = (const __attribute__((address_space(4))) int64_t *)bounds;
     s.W = W;
-    s.S = (int)s.bounds[(int64_t)W * kScheduleUnits + 1];
-    s.tail = W * (kScheduleUnits - s.S);
+    const int64_t U = (int64_t)W * kScheduleUnits;
+    s.S = (int)s.bounds[U + 1];
+    s.G = (int)s.bounds[U + 2];
+    s.tail = W * ((kScheduleUnits - s.S) / s.G);
     return s;
 }
 
-// bounds[0..waves * kScheduleUnits + 1] of the schedule above, on stream st.
+// bounds[0..waves * kScheduleUnits + 2] of the schedule above, on stream st.
 void schedule_bounds(const int64_t *cost_prefix, int64_t n, int waves, int64_t *bounds, hipStream_t st);
 
 constexpr int kMaxBins2 = 256;      // max n_2 (pi or mu bins)

@@ -354,7 +354,7 @@ struct PairArgs {
     int64_t n_items;
     const int64_t *total_cost;   // device: cost prefix total (read by the kernel)
     int n_waves;
-    const int64_t *bounds;       // schedule unit -> first item, n_waves * kScheduleUnits + 2
+    const int64_t *bounds;       // schedule unit -> first item, n_waves * kScheduleUnits + 3
     int *queue;                  // tail-chunk counter, zero at launch
     int mode, n_r, n_2, weighted, var, nbins;
     const double *T, *X;         // exact thresholds (global copy)
@@ -1127,7 +1127,7 @@ void count_fast(Ctx &ctx, std::vector<FastSample> &samples, const mia_bins &bins
     if (nitems < waves) waves = (int)std::max<int64_t>(1, nitems);
     A.n_waves = waves;
     unsigned grid = (unsigned)((waves + wpb - 1) / wpb);
-    int64_t *bounds = ctx.arena.get<int64_t>("fast/bounds", (size_t)waves * kScheduleUnits + 2);
+    int64_t *bounds = ctx.arena.get<int64_t>("fast/bounds", (size_t)waves * kScheduleUnits + 3);
     if (nitems > 0) schedule_bounds(cpre, nitems, waves, bounds, st);
     A.bounds = bounds;
     for (int pass = 0; pass < npass && nitems > 0; ++pass) {

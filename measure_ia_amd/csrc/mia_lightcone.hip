@@ -600,7 +600,7 @@ struct LcPairArgs {
     const int64_t *total_cost;
     int64_t n_items;
     int n_waves;
-    const int64_t *bounds;       // schedule unit -> first item, n_waves * kScheduleUnits + 2
+    const int64_t *bounds;       // schedule unit -> first item, n_waves * kScheduleUnits + 3
     int *queue;                  // tail-chunk counter per pass, zero at launch
     const double *prec;          // position records
     const double *sra, *sdec, *schi, *sc2w, *ss2w, *sw;
@@ -1415,7 +1415,7 @@ void count_lightcone(Ctx &ctx, const mia_lc_sample &s, const mia_bins &bins, con
     if (nitems < waves) waves = std::max(1, nitems);
     A.n_waves = waves;
     unsigned grid = (unsigned)((waves + wpb - 1) / wpb);
-    int64_t *bounds = ctx.arena.get<int64_t>("lc/bounds", (size_t)waves * kScheduleUnits + 2);
+    int64_t *bounds = ctx.arena.get<int64_t>("lc/bounds", (size_t)waves * kScheduleUnits + 3);
     if (nitems > 0) schedule_bounds(cpre, nitems, waves, bounds, st);
     A.bounds = bounds;
     for (int pass = 0; pass < npass && nitems > 0; ++pass) {

@@ -28,13 +28,18 @@ __global__ void k_fill(T *p, T v, int64_t n) {
 }
 
 // bounds[u] = first i in [0, n] with cost_prefix[i] * U >= u * C, for u in
-// [0, U]; bounds[U + 1] = the static units per wave.
+// [0, U]; bounds[U + 1], bounds[U + 2] = static units per wave and units per
+// tail chunk.
 __global__ void k_schedule_bounds(const int64_t *cpre, int64_t n, int64_t U, int waves, int64_t *bounds) {
     const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (u > U + 1) return;
+    if (u > U + 2) return;
     const double C = (double)cpre[n], Ud = (double)U, t = (double)u * C;
-    if (u == U + 1) {
-        bounds[u] = C >= kTailMinCost * waves ? kStaticUnits : kScheduleUnits;
+    if (u > U) {
+        const bool fine = C >= kTailFineCost * waves, tail = C >= kTailMinCost * waves;
+        if (u == U + 1)
+            bounds[u] = fine ? 2 : (tail ? 4 : kScheduleUnits);
+        else
+            bounds[u] = fine ? 1 : (tail ? 2 : 1);
         return;
     }
     int64_t lo = 0, hi = n;
@@ -274,7 +279,7 @@ void fill(T *p, T v, int64_t n, hipStream_t st) {
 }
 void schedule_bounds(const int64_t *cost_prefix, int64_t n, int waves, int64_t *bounds, hipStream_t st) {
     const int64_t U = (int64_t)waves * kScheduleUnits;
-    hipLaunchKernelGGL(k_schedule_bounds, dim3((unsigned)((U + 2 + 255) / 256)), dim3(256), 0, st, cost_prefix,
+    hipLaunchKernelGGL(k_schedule_bounds, dim3((unsigned)((U + 3 + 255) / 256)), dim3(256), 0, st, cost_prefix,
                        n, U, waves, bounds);
     MIA_HIP(hipGetLastError());
 }
