@@ -71,12 +71,17 @@ class CountResult:
 
 
 def _unit(w) -> bool:
+    """Whether a weight array can be dropped (unit weights).
+
+    Host arrays are checked on the host.  A device tensor is taken as given:
+    checking it would cost a reduction and a sync per count, and a weighted
+    count of unit weights gives the same DD (each product is exactly 1)."""
     if w is None:
         return True
     try:
         import torch
         if isinstance(w, torch.Tensor):
-            return bool(torch.all(w == 1).item())
+            return False
     except ImportError:
         pass
     return bool(np.all(np.asarray(w) == 1.0))

@@ -144,3 +144,37 @@ class Estimates:
             self.real_SplusD = num / (2 * self.R_jk[:, None, None])
             self.real_DD = self.DD - counts["DD_jk"]
             self.real_xi_gg = (self.real_DD / self.RR_jk) - 1
+
+
+# Lightcone estimators (measure_IA_base.py:670-743).  Each estimator is one
+# formula over named pair counts -- S+D and DD of shapes x positions, S+R of
+# shapes x random positions, SR, RD (random shapes x positions) and RR.  A
+# count that involves a random catalogue is first brought to the size of the
+# data catalogue it stands in for, by N_X / N_R_X for each random sample X it
+# contains (S for shapes, D for positions), taken in the order listed.
+LC_RANDOM_SAMPLES = {"S+D": (), "DD": (), "S+R": ("D",), "SR": ("D",), "RD": ("S",), "RR": ("S", "D")}
+
+LC_ESTIMATORS = {
+    ("clusters", "g+"): (("S+D", "DD", "S+R", "SR"), lambda c: c["S+D"] / c["DD"] - c["S+R"] / c["SR"]),
+    ("clusters", "gg"): (("DD", "RD", "SR", "RR"), lambda c: (c["DD"] - c["RD"] - c["SR"]) / c["RR"] - 1),
+    ("galaxies", "g+"): (("S+D", "S+R", "RR"), lambda c: (c["S+D"] - c["S+R"]) / c["RR"]),
+    ("galaxies", "gg"): (("DD", "RD", "SR", "RR"), lambda c: (c["DD"] - c["RD"] - c["SR"]) / c["RR"] + 1),
+}
+
+
+def lightcone_xi(estimator, part, load, num_samples):
+    """xi of one lightcone estimator ('clusters' / 'galaxies') for part 'g+' or
+    'gg'; load(name) returns the stored count of that name, num_samples the
+    catalogue sizes {"S", "D", "R_S", "R_D"}."""
+    names, formula = LC_ESTIMATORS[(estimator, part)]
+    counts = {}
+    for name in names:
+        x = load(name)
+        ratios = [num_samples[s] / num_samples["R_" + s] for s in LC_RANDOM_SAMPLES[name]]
+        if ratios:
+            f = ratios[0]
+            for r in ratios[1:]:
+                f = f * r
+            x = x * f
+        counts[name] = x
+    return formula(counts)

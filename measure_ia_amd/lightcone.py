@@ -581,96 +581,70 @@ class MeasureIALightcone(SimInfo):
 
     def _obs_estimator(self, corr_type, IA_estimator, dataset_name, dataset_name_randoms, num_samples,
                        jk_group_name="", jk_group_name_randoms=""):
-        """Cluster / galaxy estimators from the stored counts (measure_IA_base.py:670-743)."""
+        """Cluster / galaxy estimators of one realisation from its stored counts
+        (measure_IA_base.py:670-743); the formulas are estimators.LC_ESTIMATORS."""
+        if IA_estimator not in ("clusters", "galaxies"):
+            raise ValueError("Unknown input for IA_estimator, choose from [clusters, galaxies].")
+        parts = {"both": ("g+", "gg"), "g+": ("g+",), "gg": ("gg",)}.get(corr_type[0], ())
+        base = f"{self.snap_group}/{corr_type[1]}"
+        g_plus = f"{base}/xi_g_plus/{jk_group_name}"
+        gg = f"{base}/xi_gg/{jk_group_name}"
+        # (group, dataset) of every named count; the random-shape count SR has its
+        # own dataset only in a gg-only run, else it is the randoms' DD
+        where = {
+            "S+D": (g_plus, f"{dataset_name}_SplusD"),
+            "S+R": (f"{base}/xi_g_plus/{jk_group_name_randoms}", f"{dataset_name_randoms}_SplusD"),
+            "DD": (gg, f"{dataset_name}_DD"),
+            "RD": (gg, f"{dataset_name}_RD"),
+            "RR": (gg, f"{dataset_name}_RR"),
+            "SR": ((gg, f"{dataset_name}_SR") if corr_type[0] == "gg"
+                   else (f"{base}/xi_gg/{jk_group_name_randoms}", f"{dataset_name_randoms}_DD")),
+        }
+        target = {"g+": g_plus, "gg": gg}
         st, own = self._store(None)
         try:
-            base = f"{self.snap_group}/{corr_type[1]}"
-            if corr_type[0] == "g+" or corr_type[0] == "both":
-                SpD = st.read(f"{base}/xi_g_plus/{jk_group_name}/{dataset_name}_SplusD")
-                SpR = st.read(f"{base}/xi_g_plus/{jk_group_name_randoms}/{dataset_name_randoms}_SplusD")
-            gg = f"{base}/xi_gg/{jk_group_name}"
-            gg_r = f"{base}/xi_gg/{jk_group_name_randoms}"
-            DD = st.read(f"{gg}/{dataset_name}_DD")
-            if IA_estimator == "clusters":
-                if corr_type[0] == "gg":
-                    SR = st.read(f"{gg}/{dataset_name}_SR")
-                else:
-                    SR = st.read(f"{gg_r}/{dataset_name_randoms}_DD")
-                SR *= num_samples["D"] / num_samples["R_D"]
-                if corr_type[0] == "g+" or corr_type[0] == "both":
-                    SpR *= num_samples["D"] / num_samples["R_D"]
-                    st.write(f"{base}/xi_g_plus/{jk_group_name}", dataset_name, SpD / DD - SpR / SR)
-                if corr_type[0] == "gg" or corr_type[0] == "both":
-                    RD = st.read(f"{gg}/{dataset_name}_RD")
-                    RR = st.read(f"{gg}/{dataset_name}_RR")
-                    RD *= num_samples["S"] / num_samples["R_S"]
-                    RR *= (num_samples["S"] / num_samples["R_S"]) * (num_samples["D"] / num_samples["R_D"])
-                    st.write(gg, dataset_name, (DD - RD - SR) / RR - 1)
-            elif IA_estimator == "galaxies":
-                RR = st.read(f"{gg}/{dataset_name}_RR")
-                RR *= (num_samples["S"] / num_samples["R_S"]) * (num_samples["D"] / num_samples["R_D"])
-                if corr_type[0] == "g+" or corr_type[0] == "both":
-                    SpR *= num_samples["D"] / num_samples["R_D"]
-                    st.write(f"{base}/xi_g_plus/{jk_group_name}", dataset_name, (SpD - SpR) / RR)
-                if corr_type[0] == "gg" or corr_type[0] == "both":
-                    RD = st.read(f"{gg}/{dataset_name}_RD")
-                    if corr_type[0] == "gg":
-                        SR = st.read(f"{gg}/{dataset_name}_SR")
-                    else:
-                        SR = st.read(f"{gg_r}/{dataset_name_randoms}_DD")
-                    RD *= num_samples["S"] / num_samples["R_S"]
-                    SR *= num_samples["D"] / num_samples["R_D"]
-                    st.write(gg, dataset_name, (DD - RD - SR) / RR + 1)
-            else:
-                raise ValueError("Unknown input for IA_estimator, choose from [clusters, galaxies].")
+            for part in parts:
+                xi = estimators.lightcone_xi(IA_estimator, part,
+                                             lambda name: st.read("/".join(where[name])), num_samples)
+                st.write(target[part], dataset_name, xi)
         finally:
             if own:
                 st.close()
 
     def _measure_jackknife_covariance_lightcone(self, IA_estimator, corr_type, dataset_name, max_patch,
                                                 min_patch=1, randoms_suf="_randoms"):
-        """Estimators per realisation, then mean / std / covariance
-        (measure_jackknife.py:172-263)."""
-        if corr_type[0] == "both":
-            data = [corr_type[1] + "_g_plus", corr_type[1] + "_gg"]
-        elif corr_type[0] == "g+":
-            data = [corr_type[1] + "_g_plus"]
-        elif corr_type[0] == "gg":
-            data = [corr_type[1] + "_gg"]
-        else:
+        """Every leave-one-patch-out realisation's estimators and projections, then
+        mean / std / covariance of each projected correlation (measure_jackknife.py:172-263)."""
+        suffixes = {"both": ("_g_plus", "_gg"), "g+": ("_g_plus",), "gg": ("_gg",)}.get(corr_type[0])
+        if suffixes is None:
             raise KeyError("Unknown value for corr_type. Choose from [g+, gg, both]")
-        num_patches = max_patch - min_patch + 1
-        print(f"Calculating jackknife errors for {num_patches} patches for {dataset_name} with "
+        patches = range(min_patch, max_patch + 1)
+        n = len(patches)
+        print(f"Calculating jackknife errors for {n} patches for {dataset_name} with "
               f"{dataset_name}{randoms_suf} as randoms.")
+        jk = f"{dataset_name}_jk{n}"
+        project = self._measure_w_g_i if corr_type[1] == "w" else self._measure_multipoles
+        for b in patches:
+            self._obs_estimator(corr_type, IA_estimator, f"{dataset_name}_{b}", f"{dataset_name}{randoms_suf}_{b}",
+                                self.num_samples[f"{b}"], jk_group_name=jk,
+                                jk_group_name_randoms=f"{dataset_name}{randoms_suf}_jk{n}")
+            project(corr_type=corr_type[0], dataset_name=f"{dataset_name}_{b}", jk_group_name=jk)
         covs, stds = [], []
-        for d in range(len(data)):
-            for b in range(min_patch, max_patch + 1):
-                self._obs_estimator(corr_type, IA_estimator, f"{dataset_name}_{b}",
-                                    f"{dataset_name}{randoms_suf}_{b}", self.num_samples[f"{b}"],
-                                    jk_group_name=f"{dataset_name}_jk{num_patches}",
-                                    jk_group_name_randoms=f"{dataset_name}{randoms_suf}_jk{num_patches}")
-                if "w" in data[d]:
-                    self._measure_w_g_i(corr_type=corr_type[0], dataset_name=f"{dataset_name}_{b}",
-                                        jk_group_name=f"{dataset_name}_jk{num_patches}")
-                else:
-                    self._measure_multipoles(corr_type=corr_type[0], dataset_name=f"{dataset_name}_{b}",
-                                             jk_group_name=f"{dataset_name}_jk{num_patches}")
-            st, own = self._store(None)
-            try:
-                g = f"{self.snap_group}/{data[d]}/{dataset_name}_jk{num_patches}"
-                reals = [st.read(f"{g}/{dataset_name}_{b}") for b in range(min_patch, max_patch + 1)]
+        st, own = self._store(None)
+        try:
+            for suffix in suffixes:
+                group = f"{self.snap_group}/{corr_type[1]}{suffix}"
+                reals = [st.read(f"{group}/{jk}/{dataset_name}_{b}") for b in patches]
                 mean, std, cov = estimators.jackknife_combine(reals, self.num_bins_r)
-                if self.output_file_name is not None:
-                    gg = f"{self.snap_group}/{data[d]}"
-                    st.write(gg, dataset_name + "_mean_" + str(num_patches), mean)
-                    st.write(gg, dataset_name + "_jackknife_" + str(num_patches), std)
-                    st.write(gg, dataset_name + "_jackknife_cov_" + str(num_patches), cov)
-                else:
+                if self.output_file_name is None:
                     covs.append(cov)
                     stds.append(std)
-            finally:
-                if own:
-                    st.close()
+                    continue
+                for tag, value in (("mean", mean), ("jackknife", std), ("jackknife_cov", cov)):
+                    st.write(group, f"{dataset_name}_{tag}_{n}", value)
+        finally:
+            if own:
+                st.close()
         if self.output_file_name is None:
             return covs, stds
 

@@ -74,6 +74,25 @@ def make_case(name):
         pos = wrap_box(rng.uniform(0, L, (800, 3)), L)
         shp = wrap_box(rng.uniform(0, L, (500, 3)), L)
         cfg.update(r_min=0.5, r_max=4.4, nr=5, n2=6, pi_max=4.5, L_sub=2)
+    elif name == "parallel_axes":
+        # every shape's axis lies exactly along the projected separation to two
+        # partner positions (one on each side), all on an integer lattice, so
+        # d = (dx/r_p) a_x + (dy/r_p) a_y rounds to |d| > 1 for about a tenth of
+        # those pairs and the reference's arccos gives NaN -> e+ = ex = 0
+        # (measure_w_box_jk.py:166-175)
+        L = 24.0
+        shp = rng.integers(0, 24, (1500, 3)).astype(np.float64)
+        sep = np.zeros((len(shp), 3))
+        while True:
+            bad = (sep[:, 0] == 0) & (sep[:, 1] == 0)
+            if not bad.any():
+                break
+            sep[bad, :2] = rng.integers(-6, 7, (int(bad.sum()), 2))
+        sep[:, 2] = rng.integers(-5, 6, len(shp))
+        partners = np.concatenate([shp - sep, shp + sep])
+        background = rng.integers(0, 24, (500, 3)).astype(np.float64)
+        pos = wrap_box(np.concatenate([partners, background]), L)
+        cfg.update(r_min=0.5, r_max=10.0, nr=8, n2=8, pi_max=8.0, L_sub=2)
     else:
         raise KeyError(name)
     e12 = ellipticity_noise(rng, len(shp), 0.25)
@@ -81,6 +100,8 @@ def make_case(name):
     axis = axis * rng.uniform(0.5, 2.0, (len(shp), 1))  # unnormalised on input
     if name == "degenerate_axes":
         axis[rng.choice(len(shp), len(shp) // 20, replace=False)] = 0.0
+    if name == "parallel_axes":
+        axis = sep[:, :2].copy()
     data = {"Position": pos, "Position_shape_sample": shp, "Axis_Direction": axis,
             "LOS": cfg["los"], "q": q}
     if cfg["weights"]:
@@ -91,7 +112,7 @@ def make_case(name):
 
 CASES = ["uniform_jk27", "uniform_auto_nojk", "clustered_jk8", "weighted_los0", "los1_fullpi",
          "lattice_boundaries", "tiny_box_wrap", "open_box_jk27", "reach_beyond_half_box",
-         "degenerate_axes"]
+         "degenerate_axes", "parallel_axes"]
 
 
 def oracle_counts(data, L, cfg, mode):
@@ -148,9 +169,21 @@ def compare(res, ref, R, K, weighted, tol=1e-6):
     errs = dict(splus=e_sp, scross=e_sx)
     if K:
         np.testing.assert_array_equal(res.npairs[1:], ref["NP_jk"])
-        if not weighted:
+        if weighted:
+            # weighted DD tallies: sums of positive terms, plain relative error
+            e_ddjk = rel_err(res.dd[1:], ref["DD_jk"], np.abs(ref["DD_jk"])).max()
+            assert e_ddjk <= tol, ("DD_jk", e_ddjk)
+            errs["dd_jk"] = e_ddjk
+        else:
             np.testing.assert_array_equal(res.dd[1:], ref["DD_jk"])
         e_jk = rel_err(res.splus[1:], ref["SplusD_jk"], ref["SplusD_jk_abs"]).max()
         assert e_jk <= tol, ("S+D_jk", e_jk)
         errs["splus_jk"] = e_jk
     return errs
+
+
+def dyadic_weights(rng, n):
+    """Weights from {0.5, 1, 2}: every product w_p w_s and every partial sum of
+    them is exact in FP64, so weighted DD must match the oracle bit for bit
+    whatever the summation order."""
+    return rng.choice(np.array([0.5, 1.0, 2.0]), n)

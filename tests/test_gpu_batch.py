@@ -92,3 +92,44 @@ def test_batch_with_empty_shape_sample(engine):
     assert a.npairs.sum() == 0 and np.all(a.splus == 0) and np.all(a.dd == 0)
     one = engine.count(full, th)
     np.testing.assert_array_equal(b.npairs, one.npairs)
+
+
+@pytest.mark.parametrize("mode", ["rppi", "rmu"])
+def test_batch_dyadic_weights_exact(engine, mode):
+    """Weights from {0.5, 1, 2}: every DD sum is exact in FP64, so the batched
+    count, the single count and the oracle agree bit for bit on DD and on every
+    DD tally (measure_w_box_jk.py:207-211), whatever the atomic order.  A lost,
+    repeated or misattributed weighted add fails this deterministically."""
+    from paircases import dyadic_weights
+    rng = np.random.default_rng(77)
+    r_bins, pi_bins, mu_bins = prep.bin_edges(R_MIN, R_MAX, NR, NPI, PI_MAX, L)
+    if mode == "rppi":
+        th = binning.rppi_thresholds(r_bins, pi_bins, R_MIN, R_MAX)
+        sb = pi_bins
+    else:
+        th = binning.rmu_thresholds(r_bins, mu_bins, R_MIN, R_MAX)
+        sb = mu_bins
+    samples = _samples()
+    for d, _ in samples:
+        d["weight"] = dyadic_weights(rng, len(d["Position"]))
+        d["weight_shape_sample"] = dyadic_weights(rng, len(d["Position_shape_sample"]))
+    preps = [hostprep.prepare(d) for d, _ in samples]
+    es = [_engine_sample(p, jk) for p, (_, jk) in zip(preps, samples)]
+    batch = engine.count_batch(es, th)
+    singles = [engine.count(s, th, method="fast") for s in es]
+    for i, ((d, jk), p, b, one) in enumerate(zip(samples, preps, batch, singles)):
+        pr = prep.prepare(d)
+        K = jk ** 3 if jk else 0
+        jp = js = None
+        if K:
+            jp, js = prep.jackknife_region_indices(pr["positions"], pr["shapes"], L, jk)
+        ref = brute.count_pairs(pr, r_bins, sb, R_MIN, R_MAX, L, mode=mode, jk_pos=jp, jk_shape=js,
+                                num_box=K)
+        print("sample", i, len(pr["positions"]), len(pr["shapes"]), jk, flush=True)
+        np.testing.assert_array_equal(b.npairs, one.npairs)
+        np.testing.assert_array_equal(b.dd, one.dd)
+        np.testing.assert_array_equal(b.dd[0], ref["DD"])
+        if K:
+            np.testing.assert_array_equal(b.dd[1:], ref["DD_jk"])
+        compare(b, ref, p.R, K, weighted=True)
+        compare(one, ref, p.R, K, weighted=True)

@@ -220,3 +220,39 @@ def test_region_ids_by_axis_equal_the_triple_loop(n):
     np.testing.assert_array_equal(b_p, a_p)
     np.testing.assert_array_equal(b_s, a_s)
     assert (a_p > 0).sum() > 1000 or n == 1
+
+
+def test_parallel_axes_case_pins_the_arccos_nan_rule():
+    """The parallel_axes case makes the reference's d round outside [-1, 1].
+
+    measure_w_box_jk.py:166-175: d = 0 + (dx/r_p) a_x, then + (dy/r_p) a_y;
+    arccos(d) is NaN for |d| > 1 and isnan -> e+ = ex = 0.  Recomputed here
+    pair by pair with an explicit mask and the closed form e (2 d^2 - 1), the
+    oracle's S+D must agree; dropping the rule (using the closed form for
+    every pair) must not.
+    """
+    from paircases import make_case
+    data, L, cfg = make_case("parallel_axes")
+    pr = prep.prepare(data)
+    r_bins, pi_bins, _ = prep.bin_edges(cfg["r_min"], cfg["r_max"], cfg["nr"], cfg["n2"], cfg["pi_max"], L)
+    out = brute.count_pairs(pr, r_bins, pi_bins, cfg["r_min"], cfg["r_max"], L)
+    sep = pr["shapes"][None, :, :] - pr["positions"][:, None, :]
+    sep[sep > L / 2] -= L
+    sep[sep < -L / 2] += L
+    a, b, z = sep[..., pr["not_los"][0]], sep[..., pr["not_los"][1]], sep[..., pr["los"]]
+    rp = np.sqrt(a * a + b * b)
+    m = (rp >= r_bins[0]) & (rp < r_bins[-1]) & (z >= pi_bins[0]) & (z < pi_bins[-1])
+    with np.errstate(invalid="ignore", divide="ignore"):
+        d = 0.0 + (a / rp) * pr["axis"][None, :, 0]
+        d = d + (b / rp) * pr["axis"][None, :, 1]
+    nan_pairs = m & (np.abs(d) > 1.0)
+    assert nan_pairs.sum() >= 50, nan_pairs.sum()
+    e = np.broadcast_to(pr["e"][None, :], d.shape)
+    closed = e * (2 * d * d - 1)
+    keep = m & ~nan_pairs
+    with_rule = closed[keep].sum() / (2 * pr["R"])
+    without = closed[m].sum() / (2 * pr["R"])
+    got = out["SplusD"].sum()
+    scale = np.abs(closed[m]).sum() / (2 * pr["R"])
+    assert abs(got - with_rule) <= 1e-12 * scale
+    assert abs(got - without) > 1e-3 * scale
