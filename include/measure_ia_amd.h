@@ -19,9 +19,12 @@
  *   - Arrays marked "device" are HIP device pointers owned by the caller;
  *     arrays marked "host" are ordinary host memory.  The context owns all
  *     scratch memory and reuses it between calls.
- *   - Every launch goes on the caller's stream (NULL = default stream).  The
- *     count calls read a few sizes back to the host, so they synchronise
- *     that stream before returning.
+ *   - Every launch goes on the caller's stream (NULL = default stream).  A
+ *     count enqueues its whole launch sequence without reading anything back
+ *     (every size the device decides stays on the device), then reads a few
+ *     status words and synchronises that stream before returning.
+ *   - Output buffers only need to be device-accessible memory: the kernels
+ *     accumulate in the context's own workspace and write each output once.
  *   - One context per host thread; a context is not thread-safe.
  */
 #ifndef MEASURE_IA_AMD_H
@@ -33,7 +36,7 @@
 extern "C" {
 #endif
 
-#define MIA_ABI_VERSION 2
+#define MIA_ABI_VERSION 3
 
 enum mia_status {
     MIA_OK = 0,
@@ -139,6 +142,9 @@ typedef struct mia_tuning {
                                    what one wave's LDS histogram holds (large binnings
                                    are counted in passes over r-bin ranges; by default
                                    the library picks the pass size)                    */
+    int32_t graph;              /* mia_count_batch: 0 = capture the launch sequence in a
+                                   hipGraph and replay it while it stays the same (the
+                                   default); -1 = plain launches                        */
 } mia_tuning;
 
 /* Per-call statistics, filled when the pointer is non-NULL. */
@@ -154,6 +160,10 @@ typedef struct mia_stats {
     int32_t waves;
     int32_t method;         /* counting method actually used (MIA_FAST falls back to
                                MIA_PLAIN when its LDS histogram or key space is too big) */
+    double ms_total;        /* the whole call on the device (events around it)          */
+    int32_t graph;          /* 1: the call replayed a captured hipGraph                 */
+    int32_t reserved;
+    int64_t item_capacity;  /* work-item slots of the workspace                         */
 } mia_stats;
 
 int mia_abi_version(void);
@@ -171,12 +181,37 @@ int mia_count(mia_ctx *ctx, const mia_sample *sample, const mia_bins *bins,
               const mia_counts *out, int32_t method, const mia_tuning *tuning,
               mia_stats *stats, void *stream);
 
-/* Count n samples of uneven size in one persistent launch (MIA_FAST only);
- * bins are shared, outputs are per sample.  The reference has no batched
+/* Count n samples of uneven size (MIA_FAST only): all samples are sorted by
+ * one set of launches and counted by one persistent pair-kernel launch, and
+ * the whole sequence is captured in a hipGraph that later calls with the same
+ * sizes, binning and tuning replay (tuning->graph = -1: plain launches).
+ * Bins are shared, outputs are per sample.  The reference has no batched
  * form; a caller loops measure_xi_w over samples (measure_IA.py:68-163). */
 int mia_count_batch(mia_ctx *ctx, int32_t n, const mia_sample *samples, const mia_bins *bins,
                     const mia_counts *outs, const mia_tuning *tuning, mia_stats *stats,
                     void *stream);
+
+/* A sample whose cell sort is kept on the device between counts. */
+typedef struct mia_prepared mia_prepared;
+
+/* Sort one sample once (region ids, cell keys, counting sort) into memory
+ * the handle owns; the cell grid is chosen for `bins` and `tuning` and serves
+ * every later binning.  The reference rebuilds its KDTree on every counter
+ * call (measure_w_box_jk.py:386, measure_m_box_jk.py:403), so a
+ * measure_xi_w followed by measure_xi_multipoles sorts twice; with a handle
+ * the second count skips the sort.  The input arrays may be freed after the
+ * call returns.  MIA_E_RANGE as mia_count. */
+int mia_prepare_sample(mia_ctx *ctx, const mia_sample *sample, const mia_bins *bins,
+                       const mia_tuning *tuning, mia_prepared **out, void *stream);
+
+/* Count n prepared samples (MIA_FAST) in one persistent launch: the same
+ * outputs as mia_count / mia_count_batch, without the sort. */
+int mia_count_prepared(mia_ctx *ctx, int32_t n, mia_prepared *const *samples, const mia_bins *bins,
+                       const mia_counts *outs, const mia_tuning *tuning, mia_stats *stats,
+                       void *stream);
+
+/* Free a prepared sample (NULL is ignored). */
+void mia_release_prepared(mia_ctx *ctx, mia_prepared *sample);
 
 /* One lightcone measurement: a position sample and a shape sample on the sky
  * (RA, DEC in degrees, comoving distance chi), as the reference's lightcone

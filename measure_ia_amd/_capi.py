@@ -59,7 +59,7 @@ class MiaTuning(ctypes.Structure):
                 ("slabs_per_block", ctypes.c_int32), ("block_shapes", ctypes.c_int32),
                 ("item_positions", ctypes.c_int32), ("waves", ctypes.c_int32),
                 ("hist_copies", ctypes.c_int32), ("cell_shapes", ctypes.c_int32),
-                ("r_bins_per_pass", ctypes.c_int32)]
+                ("r_bins_per_pass", ctypes.c_int32), ("graph", ctypes.c_int32)]
 
 
 class MiaStats(ctypes.Structure):
@@ -69,7 +69,9 @@ class MiaStats(ctypes.Structure):
                 ("rechecks", ctypes.c_int64),
                 ("columns_per_side", ctypes.c_int32), ("slabs_per_side", ctypes.c_int32),
                 ("slabs_per_block", ctypes.c_int32), ("block_shapes", ctypes.c_int32),
-                ("waves", ctypes.c_int32), ("method", ctypes.c_int32)]
+                ("waves", ctypes.c_int32), ("method", ctypes.c_int32),
+                ("ms_total", ctypes.c_double), ("graph", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("item_capacity", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -96,6 +98,14 @@ SIGNATURES = {
     "mia_region_ids": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
                                       ctypes.c_double, ctypes.c_int32, ctypes.c_void_p,
                                       ctypes.c_void_p]),
+    "mia_prepare_sample": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(MiaSample), ctypes.POINTER(MiaBins),
+                                          ctypes.POINTER(MiaTuning), ctypes.POINTER(ctypes.c_void_p),
+                                          ctypes.c_void_p]),
+    "mia_count_prepared": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p),
+                                          ctypes.POINTER(MiaBins), ctypes.POINTER(MiaCounts),
+                                          ctypes.POINTER(MiaTuning), ctypes.POINTER(MiaStats),
+                                          ctypes.c_void_p]),
+    "mia_release_prepared": (None, [ctypes.c_void_p, ctypes.c_void_p]),
 }
 
 _lib = None

@@ -7,6 +7,7 @@
 struct mia_ctx {
     mia::Ctx c;
 };
+struct mia_prepared;
 
 namespace {
 
@@ -96,6 +97,14 @@ int mia_create(int32_t device, mia_ctx **out) {
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
         cus > 0)
         c->c.cu_count = cus;
+    try {
+        mia::stage_init(c->c);
+    } catch (const mia::Error &e) {
+        g_create_error = e.msg;
+        mia::stage_release(c->c);
+        delete c;
+        return e.code;
+    }
     *out = c;
     return MIA_OK;
 }
@@ -103,6 +112,7 @@ int mia_create(int32_t device, mia_ctx **out) {
 void mia_destroy(mia_ctx *ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->c.device);
+    mia::stage_release(ctx->c);
     delete ctx;
 }
 
@@ -126,39 +136,51 @@ int mia_count(mia_ctx *ctx, const mia_sample *sample, const mia_bins *bins, cons
         // plain kernel, which has no LDS histogram or planner key space
         if (method == MIA_FAST && !mia::fast_path_fits(*bins, weighted, sample->jk_per_side))
             method = MIA_PLAIN;
+        if (method == MIA_FAST) {
+            mia::stage_count(ctx->c, 1, sample, *bins, out, tuning, stats, st, mia::STAGE_EAGER);
+            return;
+        }
+        // the plain comparator: global-atomic histograms in context memory,
+        // copied to the caller's outputs at the end
         mia::Grid g = mia::make_grid(*sample, *bins, tuning);
+        const int K = g.jk > 0 ? g.jk * g.jk * g.jk : 0;
+        const int64_t B = (int64_t)bins->n_r * bins->n_2, nb = (1 + K) * B;
+        double *acc = ctx->c.arena.get<double>("plain/acc", (size_t)(3 * nb + 2 * B + 4 * K));
+        mia_counts a{};
+        a.npairs = (uint64_t *)acc;
+        a.dd = acc + nb;
+        a.splus = acc + 2 * nb;
+        a.scross = acc + 3 * nb;
+        a.variance = out->variance ? acc + 3 * nb + B : nullptr;
+        a.region_stats = out->region_stats ? acc + 3 * nb + 2 * B : nullptr;
+        mia::fill<double>(acc, 0.0, 3 * nb + 2 * B + 4 * K, st);
         mia::Sorted pos, shp;
         int *oor = ctx->c.arena.get<int>("oor", 1);
         mia::Events ev(2);
         MIA_HIP(hipEventRecord(ev[0], st));
         mia::fill<int32_t>(oor, 0, 1, st);
-        mia::prepare_sample(ctx->c, *sample, g, pos, shp, out->region_stats, oor,
-                            method == MIA_PLAIN, st, "s0");
+        mia::prepare_sample(ctx->c, *sample, g, pos, shp, a.region_stats, oor, true, st, "s0");
         MIA_HIP(hipEventRecord(ev[1], st));
-        // the fast path reads the range flag with its first readback
-        if (method == MIA_PLAIN) mia::check_prepared(oor, 1, st);
-        int K = g.jk > 0 ? g.jk * g.jk * g.jk : 0;
-        int64_t nb = (int64_t)(1 + K) * bins->n_r * bins->n_2;
-        if (method == MIA_PLAIN) {
-            mia::count_plain(ctx->c, *sample, *bins, g, pos, shp, *out, weighted, st, stats);
-        } else {
-            std::vector<mia::FastSample> v(1);
-            v[0].s = sample;
-            v[0].g = g;
-            v[0].pos = pos;
-            v[0].shp = shp;
-            v[0].out = out;
-            v[0].weighted = weighted;
-            mia::count_fast(ctx->c, v, *bins, tuning, st, stats, oor);
-        }
-        unit_dd(*out, weighted, nb, st);
+        mia::check_prepared(oor, 1, st);
+        mia::count_plain(ctx->c, *sample, *bins, g, pos, shp, a, weighted, st, stats);
+        unit_dd(a, weighted, nb, st);
+        auto copy = [&](void *dst, const void *src, int64_t words) {
+            if (dst && words > 0)
+                MIA_HIP(hipMemcpyAsync(dst, src, 8 * words, hipMemcpyDeviceToDevice, st));
+        };
+        copy(out->npairs, a.npairs, nb);
+        copy(out->dd, a.dd, nb);
+        copy(out->splus, a.splus, nb);
+        copy(out->scross, a.scross, B);
+        copy(out->variance, a.variance, B);
+        copy(out->region_stats, a.region_stats, 4 * K);
         MIA_HIP(hipStreamSynchronize(st));
         float ms = ev.ms(0, 1);
         if (stats) {
             stats->ms_prepare = ms;
             stats->columns_per_side = g.nc;
             stats->slabs_per_side = g.nz;
-            stats->method = method;
+            stats->method = MIA_PLAIN;
         }
     });
 }
@@ -168,15 +190,11 @@ int mia_count_batch(mia_ctx *ctx, int32_t n, const mia_sample *samples, const mi
                     void *stream) {
     return guard(ctx, [&] {
         MIA_REQUIRE(n >= 1 && samples && outs, "empty batch");
+        MIA_REQUIRE(n <= 4096, "more than 4096 samples in one batch");
         check_bins(bins);
         MIA_HIP(hipSetDevice(ctx->c.device));
         hipStream_t st = (hipStream_t)stream;
         if (stats) std::memset(stats, 0, sizeof(*stats));
-        std::vector<mia::FastSample> v(n);
-        int *oor = ctx->c.arena.get<int>("oor", n);
-        mia::Events ev(2);
-        MIA_HIP(hipEventRecord(ev[0], st));
-        mia::fill<int32_t>(oor, 0, n, st);
         for (int i = 0; i < n; ++i) {
             check_sample(&samples[i]);
             bool weighted = samples[i].pos_w != nullptr || samples[i].shp_w != nullptr;
@@ -185,28 +203,49 @@ int mia_count_batch(mia_ctx *ctx, int32_t n, const mia_sample *samples, const mi
                 throw mia::Error{MIA_E_UNSUPPORTED,
                                  "binning or jackknife count too large for the batched kernel; "
                                  "count the samples one by one (mia_count falls back to MIA_PLAIN)"};
-            v[i].s = &samples[i];
-            v[i].g = mia::make_grid(samples[i], *bins, tuning);
-            v[i].out = &outs[i];
-            v[i].weighted = weighted;
-            mia::prepare_sample(ctx->c, samples[i], v[i].g, v[i].pos, v[i].shp,
-                                outs[i].region_stats, oor + i, false, st, "s" + std::to_string(i));
         }
-        MIA_HIP(hipEventRecord(ev[1], st));
-        mia::count_fast(ctx->c, v, *bins, tuning, st, stats, oor);
-        for (int i = 0; i < n; ++i) {
-            int K = v[i].g.jk > 0 ? v[i].g.jk * v[i].g.jk * v[i].g.jk : 0;
-            unit_dd(outs[i], v[i].weighted, (int64_t)(1 + K) * bins->n_r * bins->n_2, st);
-        }
-        MIA_HIP(hipStreamSynchronize(st));
-        float ms = ev.ms(0, 1);
-        if (stats) {
-            stats->ms_prepare = ms;
-            stats->columns_per_side = v[0].g.nc;
-            stats->slabs_per_side = v[0].g.nz;
-            stats->method = MIA_FAST;
-        }
+        const bool graph = !(tuning && tuning->graph < 0);
+        mia::stage_count(ctx->c, n, samples, *bins, outs, tuning, stats, st,
+                         graph ? mia::STAGE_GRAPH : mia::STAGE_EAGER);
     });
+}
+
+int mia_prepare_sample(mia_ctx *ctx, const mia_sample *sample, const mia_bins *bins,
+                       const mia_tuning *tuning, mia_prepared **out, void *stream) {
+    return guard(ctx, [&] {
+        MIA_REQUIRE(out != nullptr, "out is NULL");
+        *out = nullptr;
+        check_sample(sample);
+        check_bins(bins);
+        MIA_HIP(hipSetDevice(ctx->c.device));
+        *out = (mia_prepared *)mia::stage_prepare(ctx->c, *sample, *bins, tuning, (hipStream_t)stream);
+    });
+}
+
+int mia_count_prepared(mia_ctx *ctx, int32_t n, mia_prepared *const *samples, const mia_bins *bins,
+                       const mia_counts *outs, const mia_tuning *tuning, mia_stats *stats,
+                       void *stream) {
+    return guard(ctx, [&] {
+        MIA_REQUIRE(n >= 1 && samples && outs, "empty batch");
+        MIA_REQUIRE(n <= 4096, "more than 4096 samples in one batch");
+        check_bins(bins);
+        for (int i = 0; i < n; ++i) {
+            MIA_REQUIRE(samples[i] != nullptr, "prepared sample is NULL");
+            const mia::SortedSample &x = mia::prepared_view((const mia::PreparedSample *)samples[i]);
+            check_out(&outs[i], x.weighted != 0);
+            if (!mia::fast_path_fits(*bins, true, x.g.jk))
+                throw mia::Error{MIA_E_UNSUPPORTED, "binning or jackknife count too large for MIA_FAST"};
+        }
+        MIA_HIP(hipSetDevice(ctx->c.device));
+        if (stats) std::memset(stats, 0, sizeof(*stats));
+        mia::stage_count_prepared(ctx->c, n, (const mia::PreparedSample *const *)samples, *bins, outs, tuning,
+                                  stats, (hipStream_t)stream);
+    });
+}
+
+void mia_release_prepared(mia_ctx *ctx, mia_prepared *sample) {
+    if (ctx) (void)hipSetDevice(ctx->c.device);
+    mia::stage_free_prepared((mia::PreparedSample *)sample);
 }
 
 int mia_count_lightcone(mia_ctx *ctx, const mia_lc_sample *s, const mia_bins *bins,

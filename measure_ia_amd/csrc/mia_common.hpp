@@ -94,6 +94,9 @@ __device__ __forceinline__ WaveSchedule make_schedule(const int64_t *bounds, int
 
 // bounds[0..waves * kScheduleUnits + 2] of the schedule above, on stream st.
 void schedule_bounds(const int64_t *cost_prefix, int64_t n, int waves, int64_t *bounds, hipStream_t st);
+// the same with the item count read on the device
+void schedule_bounds(const int64_t *cost_prefix, const int64_t *n_dev, int waves, int64_t *bounds,
+                     hipStream_t st);
 
 constexpr int kMaxBins2 = 256;      // max n_2 (pi or mu bins)
 constexpr int kMaxBinsR = 256;      // max n_r
@@ -194,6 +197,7 @@ class Events {
     Events(const Events &) = delete;
     Events &operator=(const Events &) = delete;
     hipEvent_t operator[](int i) const { return ev_[i]; }
+    const hipEvent_t *data() const { return ev_.data(); }
     // milliseconds between two recorded events (the later one must be complete)
     float ms(int a, int b) const {
         float t = 0;
@@ -220,19 +224,94 @@ void count_plain(Ctx &ctx, const mia_sample &s, const mia_bins &bins, const Grid
                  const Sorted &pos, const Sorted &shp, const mia_counts &out, bool weighted,
                  hipStream_t st, mia_stats *stats);
 
-struct FastSample {
-    const mia_sample *s;
-    Grid g;
-    Sorted pos, shp;
-    const mia_counts *out;
-    bool weighted;
-};
 // Whether the batched kernel can count this binning: one wave's histogram
 // must fit in LDS and the (1+K)^2 planner keys must stay bounded.
 bool fast_path_fits(const mia_bins &bins, bool weighted, int jk);
-// oor: device out-of-box flags of the samples (checked with the first readback), or NULL
-void count_fast(Ctx &ctx, std::vector<FastSample> &samples, const mia_bins &bins,
-                const mia_tuning *tuning, hipStream_t st, mia_stats *stats, const int *oor);
+
+// ---------------------------------------------------------------------------
+// Staged fast count.  Every size the device decides (shape blocks, work
+// items, cost totals) stays on the device, all scratch is carved from one
+// workspace whose size the host bounds beforehand, and no stage reads
+// anything back.  The whole sequence can therefore be replayed from a
+// hipGraph, or run inside a fixed (static) workspace.
+
+// A cell-sorted sample on the device: a prepared handle, or one sample of a
+// batch prepared inside the workspace.
+struct SortedSample {
+    int64_t n_pos = 0, n_shape = 0;
+    Grid g;
+    int K = 0;                        // jackknife regions (0: none)
+    int periodic = 1;
+    int weighted = 0;                 // a weight array was given
+    const double *pa = nullptr, *pb = nullptr, *pl = nullptr, *pw = nullptr;  // positions (a, b, los, w)
+    const double *srec = nullptr;     // 8 doubles per shape {a, b, l, ax, ay, e*w, w, 2*e*w}
+    const int32_t *pstart = nullptr, *sstart = nullptr;  // keys + 1 start offsets into the arrays above
+    const unsigned long long *rcnt = nullptr;  // [2K] per region: positions, shapes
+    const double *rsum = nullptr;     // [2K] per region: sum w_s, sum w_s (1 - e^2/2)
+};
+
+// Bytes a workspace needs, and the carving of it (mia_stage.hip).
+struct Carve {
+    char *base = nullptr;
+    size_t off = 0;
+    bool dry = true;  // dry run: only sum the sizes
+    template <class T>
+    T *take(size_t count) {
+        size_t b = (count * sizeof(T) + 255) & ~(size_t)255;
+        if (b == 0) b = 256;
+        T *p = dry ? nullptr : reinterpret_cast<T *>(base + off);
+        off += b;
+        return p;
+    }
+};
+
+// Scratch of one exclusive scan of up to `cap` elements (tile sums per level).
+template <class T>
+struct ScanBufs {
+    int levels = 0;
+    int64_t n[4] = {0, 0, 0, 0};      // elements per level (level 0 = the input)
+    T *sums[4] = {nullptr, nullptr, nullptr, nullptr};
+    T *offs[4] = {nullptr, nullptr, nullptr, nullptr};
+};
+template <class T>
+void scan_carve(Carve &c, int64_t cap, ScanBufs<T> &b);
+// Exclusive scan of in[0..n) into out[0..n], out[n] = total.  n = min(*n_dev,
+// cap) when n_dev is given (read on the device), else cap.
+template <class T>
+void scan_ws(const T *in, T *out, int64_t cap, const int64_t *n_dev, const ScanBufs<T> &b,
+             hipStream_t st);
+
+// Per-call parameters of the count stages, uploaded once per call.
+struct CountCall;
+// Host plan of a staged count over n sorted samples: worst-case sizes,
+// launch shapes and the pair-kernel configuration.
+struct CountPlan;
+struct CountPlan *plan_count(Ctx &ctx, const std::vector<SortedSample> &ss, const mia_bins &bins,
+                             const mia_tuning *tuning, int64_t item_cap);
+void free_plan(CountPlan *p);
+// workspace bytes of the count stages (after plan_count)
+size_t count_bytes(const CountPlan &p);
+// Carves the count stages' buffers from c and writes the per-call parameter
+// block (host image) for these samples and outputs.
+void count_bind(CountPlan &p, Carve &c, const std::vector<SortedSample> &ss, const mia_counts *outs);
+// host bytes of the parameter block and its image
+size_t count_param_bytes(const CountPlan &p);
+const void *count_param_image(const CountPlan &p);
+void *count_param_dev(const CountPlan &p);
+// Enqueues every count stage on st (no host sync, no allocation).  With ev
+// (4 events), records them before the plan, before the pairs, after the
+// pairs and at the end.
+void count_enqueue(const CountPlan &p, hipStream_t st, const hipEvent_t *ev);
+// Device words read back after a count: [0] rechecks, [1] candidates,
+// [2] item overflow, [3] items, [4] pair-kernel start (~wall clock), [5] end.
+const unsigned long long *count_readback(const CountPlan &p);
+constexpr int kCountReadback = 6;
+int count_waves(const CountPlan &p);
+int count_bsz(const CountPlan &p);
+int count_G(const CountPlan &p, int sample);
+int64_t count_item_cap(const CountPlan &p);
+// a key identifying the launch sequence (everything but caller pointers)
+std::string count_key(const CountPlan &p);
 
 // Lightcone counter (mia_lightcone.hip): whether the binning's histogram fits
 // one wave's LDS share, and the count itself (outputs zeroed and filled).
@@ -246,7 +325,35 @@ struct Ctx {
     std::string last_error;
     Arena arena;
     int cu_count = 256;
+    // staged counts (mia_stage.hip)
+    void *ws = nullptr;             // growable workspace
+    size_t ws_bytes = 0;
+    void *pinned = nullptr;         // host staging of parameter blocks and read-backs
+    size_t pinned_bytes = 0;
+    hipStream_t cap_stream = nullptr;   // stream the batched count is captured on
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    std::string gkey;               // launch-sequence key of gexec
+    int64_t item_cap = 0;           // grown after an item overflow
+    double wall_khz = 100000.0;     // device wall-clock rate
+    void *scratch = nullptr;        // the device's static workspace (allocation-free entry point)
 };
+
+// Staged-count resources of a context (mia_stage.hip).
+void stage_init(Ctx &ctx);
+void stage_release(Ctx &ctx);
+struct PreparedSample;
+// mia_count (MIA_FAST), mia_count_batch, mia_count_noalloc and the handle calls.
+enum StageMode { STAGE_EAGER = 0, STAGE_GRAPH = 1, STAGE_STATIC = 2 };
+void stage_count(Ctx &ctx, int n, const mia_sample *samples, const mia_bins &bins, const mia_counts *outs,
+                 const mia_tuning *tuning, mia_stats *stats, hipStream_t st, StageMode mode);
+PreparedSample *stage_prepare(Ctx &ctx, const mia_sample &s, const mia_bins &bins, const mia_tuning *tuning,
+                              hipStream_t st);
+void stage_count_prepared(Ctx &ctx, int n, const PreparedSample *const *h, const mia_bins &bins,
+                          const mia_counts *outs, const mia_tuning *tuning, mia_stats *stats, hipStream_t st);
+void stage_free_prepared(PreparedSample *h);
+const SortedSample &prepared_view(const PreparedSample *h);
+int64_t stage_static_bytes();
 
 template <class T>
 void fill(T *p, T v, int64_t n, hipStream_t st);

@@ -30,6 +30,8 @@
 #include "mia_common.hpp"
 
 #include <cmath>
+#include <memory>
+#include <mutex>
 
 namespace mia {
 
@@ -65,32 +67,51 @@ struct SampleDev {
     const int32_t *pstart;
     double L, half;
     int los, K, nbins, pad;
-    // accumulators (the caller's outputs)
+    // accumulators (workspace)
     unsigned long long *npairs;   // (1+K)*B
-    double *dd, *sp;              // (1+K)*B
+    double *dd, *sp;              // (1+K)*B (dd: weighted samples only)
     double *sx, *var;             // B
 };
 
-struct PlanArgs {
+// Planner view of one sample (per-call parameter block).  Its (column, slab
+// group) cells occupy [cell0, cell0 + ncells) of the global cell arrays, and
+// its plan threads [t0, t0 + nt) with t0 a multiple of kPlanThreads, so a
+// workgroup never straddles two samples.  nt = worst-case blocks x nxw^2.
+struct PlanSample {
     Grid g;
-    int mode, periodic, sample, K1;
+    int mode, periodic;
     int ng, G;           // slab groups per column, slabs per group
     int kx, nxw;         // stencil half width / width in columns
-    int pmax;
+    int pmax;            // positions per work item at most
+    int pad;
     double reach_r, reach_z;
     const int32_t *sstart, *pstart;
-    const Block *blocks;
-    int nblocks;
-    int blk_base;        // global index of this sample's first block
-    unsigned long long *key_count;   // count pass
-    unsigned long long *key_cursor;  // emit pass
-    const int64_t *key_off;
-    Item *items;
-    int64_t *cost;       // emit pass: balancing cost of every item
-    unsigned long long *candidates;  // emit pass: sum of ns * np
-    int emit;
-    int local;           // 1: aggregate per workgroup in LDS (K1^2 keys fit)
+    int64_t cell0, ncells;
+    int64_t t0, nt;
 };
+
+// Finalisation view of one sample: accumulators to caller outputs.
+struct FinSample {
+    const unsigned long long *np;
+    const double *sp, *dd, *sx, *var;
+    const unsigned long long *rcnt;   // [2K] objects per region (positions, shapes)
+    const double *rsum;               // [2K] sum w_s, sum w_s (1 - e^2/2) per region
+    mia_counts out;
+    int K, B, weighted, var_on;
+};
+
+// Largest s in [0, n) with tab[s] <= v (tab ascending, tab[0] <= v).
+__device__ __forceinline__ int find_sample(const int64_t *tab, int n, int64_t v) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tab[mid] <= v)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
 
 __device__ __forceinline__ int region_of(const Grid &g, int ia, int ib, int il) {
     if (g.jk <= 0) return 0;
@@ -140,15 +161,13 @@ __device__ __forceinline__ void axis_class(double lo, double hi, double L, doubl
 // Enumerate the position ranges of one (block, stencil column) pair: slab
 // runs within reach of the block, merged while the periodic shift class and
 // the jackknife slab stay the same, plus the column's anomalous tail.  For
-// each range calls emit(local key, p_begin, p_end, rp, shift byte, block, ns)
-// with local key = R_s * K1 + R_p.
+// each range calls emit(local key, p_begin, p_end, rp, shift byte) with
+// local key = R_s * K1 + R_p.
 template <class F>
-__device__ __forceinline__ void plan_ranges(const PlanArgs &P, int64_t t, F &&emit) {
+__device__ __forceinline__ void plan_ranges(const PlanSample &P, const Block &B, int so, int K1,
+                                            F &&emit) {
     const Grid &g = P.g;
-    int bl = (int)(t / (P.nxw * P.nxw));
-    int so = (int)(t % (P.nxw * P.nxw));
     int ox = so / P.nxw, oy = so % P.nxw;
-    Block B = P.blocks[bl];
     int ia = B.col / g.nc, ib = B.col % g.nc;
     bool full = P.nxw >= g.nc;
     int jx = full ? ox : ia - P.kx + ox;
@@ -182,8 +201,7 @@ __device__ __forceinline__ void plan_ranges(const PlanArgs &P, int64_t t, F &&em
     }
     int col = jx * g.nc + jy;
     int64_t base = (int64_t)col * (g.nz + 1);
-    const int blk = P.blk_base + bl;
-    const int kr = B.rs * P.K1;
+    const int kr = B.rs * K1;
     auto range = [&](int p_begin, int p_end, int rp, int cz) {
         if (p_end <= p_begin) return;
         // mixed shift classes take the per-pair minimum-image wrap; a
@@ -192,7 +210,7 @@ __device__ __forceinline__ void plan_ranges(const PlanArgs &P, int64_t t, F &&em
         bool wrap = P.periodic && ((ca == 3) || (cb == 3) || (cz == 3));
         uint8_t sh = (uint8_t)((ca == 3 ? 0 : ca) | ((cb == 3 ? 0 : cb) << 2) |
                                ((cz == 3 ? 0 : cz) << 4) | (wrap ? 64 : 0));
-        emit(kr + rp, p_begin, p_end, rp, sh, blk, B.ns);
+        emit(kr + rp, p_begin, p_end, rp, sh);
     };
     // walk slabs, merging consecutive included slabs with equal (shift class, jk slab)
     int run_k0 = -1, run_cls = 0, run_jz = 0;
@@ -223,124 +241,206 @@ __device__ __forceinline__ void plan_ranges(const PlanArgs &P, int64_t t, F &&em
 }
 
 constexpr int kPlanThreads = 256;
+constexpr int kPlanGridPerCu = 16;      // workgroups per CU of the planner's persistent grid
 constexpr int64_t kItemOverhead = 256;  // per-item cost, in pair evaluations
 
+struct PlanLaunch {
+    const PlanSample *ps;
+    const int64_t *t0;        // [S + 1] first plan thread of each sample (t0[S]: total)
+    int S;
+    int K1;                   // regions + 1 of the largest sample's key space (keys per sample K1^2)
+    const int32_t *boff;      // block offset of every cell (exclusive scan of the block counts)
+    const Block *blocks;
+    unsigned long long *key_count;   // count pass
+    unsigned long long *key_cursor;  // emit pass
+    const int64_t *key_off;
+    Item *items;
+    int64_t *cost;            // emit pass: balancing cost of every item
+    int64_t cap;              // item capacity
+    unsigned long long *overflow;  // set when the items exceed the capacity
+    unsigned long long *candidates;  // emit pass: sum of ns * np
+    int emit;
+    int local;                // 1: aggregate per workgroup in LDS (K1^2 keys fit)
+};
+
 // Count (emit == 0) or emit the work items of every (block, stencil column)
-// pair, counting-sorted by key (sample, R_s, R_p).  With `local` the key
-// counts are aggregated per workgroup in LDS, so each key sees one global
-// atomic per workgroup instead of one per range; the emit pass enumerates the
-// ranges twice (count, then place) around that reservation.
-__global__ __launch_bounds__(kPlanThreads) void k_plan_items(PlanArgs P) {
+// pair of every sample, counting-sorted by key (sample, R_s, R_p).  A
+// persistent grid walks the virtual workgroups of the samples' worst-case
+// thread ranges and skips those past a sample's actual block count (known
+// only on the device).  With `local` the key counts are aggregated per
+// workgroup in LDS, so each key sees one global atomic per workgroup instead
+// of one per range; the emit pass enumerates the ranges twice (count, then
+// place) around that reservation.  Items beyond the capacity are not written
+// and raise the overflow flag (the host then grows the capacity and recounts).
+__global__ __launch_bounds__(kPlanThreads) void k_plan_items(PlanLaunch A) {
     // local: reserved bases [nk] (64-bit), then counts [nk]
     extern __shared__ int64_t lmem[];
-    const int nk = P.K1 * P.K1;
+    const int nk = A.K1 * A.K1;
     int64_t *lbase = lmem;
     int32_t *lcnt = (int32_t *)(lmem + nk);
-    const int kb = P.sample * nk;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool active = t < (int64_t)P.nblocks * P.nxw * P.nxw;
-    const bool local = P.local != 0;
-    const int pmax = P.pmax;
-    if (local) {
-        for (int j = threadIdx.x; j < nk; j += blockDim.x) {
-            lcnt[j] = 0;
-            lbase[j] = 0;
-        }
-        __syncthreads();
-    }
-    if (!P.emit || local) {
-        if (active)
-            plan_ranges(P, t, [&](int k, int pb, int pe, int, uint8_t, int, int) {
-                int n = (pe - pb + pmax - 1) / pmax;
-                if (local)
-                    atomicAdd(&lcnt[k], n);
-                else
-                    atomicAdd(&P.key_count[kb + k], (unsigned long long)n);
-            });
-    }
-    if (local) {
-        __syncthreads();
-        for (int j = threadIdx.x; j < nk; j += blockDim.x) {
-            int c = lcnt[j];
-            if (!P.emit) {
-                if (c) atomicAdd(&P.key_count[kb + j], (unsigned long long)c);
-            } else {
-                lbase[j] = c ? (int64_t)atomicAdd(&P.key_cursor[kb + j], (unsigned long long)c) : 0;
-                lcnt[j] = 0;
-            }
-        }
-        __syncthreads();
-    }
-    if (!P.emit) return;
     __shared__ unsigned long long cand_l;
+    const int64_t nvwg = (A.t0[A.S] + kPlanThreads - 1) / kPlanThreads;
+    const bool local = A.local != 0;
+    unsigned long long cand = 0;
+    for (int64_t vb = blockIdx.x; vb < nvwg; vb += gridDim.x) {
+        const int s = find_sample(A.t0, A.S, vb * kPlanThreads);
+        const PlanSample &P = A.ps[s];
+        const int64_t tl0 = vb * kPlanThreads - P.t0;
+        const int nx2 = P.nxw * P.nxw;
+        const int b0 = A.boff[P.cell0], b1 = A.boff[P.cell0 + P.ncells];
+        // the whole virtual workgroup lies past the sample's blocks
+        if (tl0 >= P.nt || b0 + tl0 / nx2 >= b1) continue;
+        const int64_t tl = tl0 + threadIdx.x;
+        const int blk = b0 + (int)(tl / nx2);
+        const bool active = tl < P.nt && blk < b1;
+        const int so = (int)(tl % nx2);
+        const int kb = s * nk;
+        const int pmax = P.pmax;
+        Block B{};
+        if (active) B = A.blocks[blk];
+        if (local) {
+            __syncthreads();  // the previous virtual workgroup is done with the LDS tables
+            for (int j = threadIdx.x; j < nk; j += blockDim.x) {
+                lcnt[j] = 0;
+                lbase[j] = 0;
+            }
+            __syncthreads();
+        }
+        if (!A.emit || local) {
+            if (active)
+                plan_ranges(P, B, so, A.K1, [&](int k, int pb, int pe, int, uint8_t) {
+                    int n = (pe - pb + pmax - 1) / pmax;
+                    if (local)
+                        atomicAdd(&lcnt[k], n);
+                    else
+                        atomicAdd(&A.key_count[kb + k], (unsigned long long)n);
+                });
+        }
+        if (local) {
+            __syncthreads();
+            for (int j = threadIdx.x; j < nk; j += blockDim.x) {
+                int c = lcnt[j];
+                if (!A.emit) {
+                    if (c) atomicAdd(&A.key_count[kb + j], (unsigned long long)c);
+                } else {
+                    lbase[j] = c ? (int64_t)atomicAdd(&A.key_cursor[kb + j], (unsigned long long)c) : 0;
+                    lcnt[j] = 0;
+                }
+            }
+            __syncthreads();
+        }
+        if (!A.emit) continue;
+        if (active) plan_ranges(P, B, so, A.K1, [&](int k, int pb, int pe, int rp, uint8_t sh) {
+            int n = (pe - pb + pmax - 1) / pmax;
+            int64_t slot = A.key_off[kb + k] +
+                           (local ? lbase[k] + (int64_t)atomicAdd(&lcnt[k], n)
+                                  : (int64_t)atomicAdd(&A.key_cursor[kb + k], (unsigned long long)n));
+            if (slot + n > A.cap) {
+                atomicOr(A.overflow, 1ull);
+                return;
+            }
+            for (int q = 0; q < n; ++q) {
+                int p = pb + q * pmax;
+                Item it;
+                it.blk = blk;
+                it.p0 = p;
+                it.np = pe - p < pmax ? pe - p : pmax;
+                it.rp = (int16_t)rp;
+                it.shifts = sh;
+                it.pad = 0;
+                A.items[slot + q] = it;
+                // a wave pays for whole 64-lane position batches plus a fixed
+                // per-item overhead, whatever the item's true pair count
+                A.cost[slot + q] = (int64_t)B.ns * (((it.np + kWave - 1) / kWave) * kWave) + kItemOverhead;
+                cand += (unsigned long long)B.ns * it.np;
+            }
+        });
+    }
+    if (!A.emit) return;
     if (threadIdx.x == 0) cand_l = 0;
     __syncthreads();
-    unsigned long long cand = 0;
-    if (active) plan_ranges(P, t, [&](int k, int pb, int pe, int rp, uint8_t sh, int blk, int ns) {
-        int n = (pe - pb + pmax - 1) / pmax;
-        int64_t slot = P.key_off[kb + k] +
-                       (local ? lbase[k] + (int64_t)atomicAdd(&lcnt[k], n)
-                              : (int64_t)atomicAdd(&P.key_cursor[kb + k], (unsigned long long)n));
-        for (int q = 0; q < n; ++q) {
-            int p = pb + q * pmax;
-            Item it;
-            it.blk = blk;
-            it.p0 = p;
-            it.np = pe - p < pmax ? pe - p : pmax;
-            it.rp = (int16_t)rp;
-            it.shifts = sh;
-            it.pad = 0;
-            P.items[slot + q] = it;
-            // a wave pays for whole 64-lane position batches plus a fixed
-            // per-item overhead, whatever the item's true pair count
-            P.cost[slot + q] = (int64_t)ns * (((it.np + kWave - 1) / kWave) * kWave) + kItemOverhead;
-            cand += (unsigned long long)ns * it.np;
-        }
-    });
     if (cand) atomicAdd(&cand_l, cand);
     __syncthreads();
-    if (threadIdx.x == 0 && cand_l) atomicAdd(P.candidates, cand_l);
+    if (threadIdx.x == 0 && cand_l) atomicAdd(A.candidates, cand_l);
 }
 
-__global__ void k_blocks_count(const int32_t *sstart, Grid g, int ng, int G, int bsz,
-                               int32_t *nblk) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    int ncol = g.nc * g.nc;
-    if (t >= ncol * (ng + 1)) return;
-    int col = t / (ng + 1), grp = t % (ng + 1);
-    int64_t base = (int64_t)col * (g.nz + 1);
-    int c = grp < ng ? sstart[base + (grp + 1) * G] - sstart[base + grp * G]
-                     : sstart[base + g.nz + 1] - sstart[base + g.nz];
-    nblk[t] = (c + bsz - 1) / bsz;
-}
-
-__global__ void k_blocks_emit(const int32_t *sstart, Grid g, int ng, int G, int bsz,
-                              const int32_t *off, int sample, Block *blocks) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    int ncol = g.nc * g.nc;
-    if (t >= ncol * (ng + 1)) return;
-    int col = t / (ng + 1), grp = t % (ng + 1);
-    int64_t base = (int64_t)col * (g.nz + 1);
-    int s0, c, rs;
-    if (grp < ng) {
-        s0 = sstart[base + grp * G];
-        c = sstart[base + (grp + 1) * G] - s0;
-        rs = region_of(g, col / g.nc, col % g.nc, grp * G);
+// Shape blocks: per (column, slab group) cell of every sample, the number of
+// blocks of at most bsz shapes (count), then the blocks themselves at the
+// scanned offsets (emit).  cell0[S + 1] locates each sample's cells.
+__global__ void k_blocks(const PlanSample *ps, const int64_t *cell0, int S, int bsz,
+                         int32_t *nb, const int32_t *boff, Block *blocks) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= cell0[S]) return;
+    const int s = find_sample(cell0, S, c);
+    const PlanSample &P = ps[s];
+    const Grid &g = P.g;
+    const int64_t local = c - P.cell0;
+    const int col = (int)(local / (P.ng + 1)), grp = (int)(local % (P.ng + 1));
+    const int64_t base = (int64_t)col * (g.nz + 1);
+    int s0, n, rs;
+    if (grp < P.ng) {
+        s0 = P.sstart[base + grp * P.G];
+        n = P.sstart[base + (grp + 1) * P.G] - s0;
+        rs = region_of(g, col / g.nc, col % g.nc, grp * P.G);
     } else {
-        s0 = sstart[base + g.nz];
-        c = sstart[base + g.nz + 1] - s0;
+        s0 = P.sstart[base + g.nz];
+        n = P.sstart[base + g.nz + 1] - s0;
         rs = 0;
     }
-    int o = off[t];
-    for (int b = 0; b * bsz < c; ++b) {
+    if (!boff) {
+        nb[c] = (n + bsz - 1) / bsz;
+        return;
+    }
+    const int o = boff[c];
+    for (int b = 0; b * bsz < n; ++b) {
         Block B;
         B.s0 = s0 + b * bsz;
-        B.ns = c - b * bsz < bsz ? c - b * bsz : bsz;
+        B.ns = n - b * bsz < bsz ? n - b * bsz : bsz;
         B.col = col;
         B.grp = grp;
-        B.sample = sample;
+        B.sample = s;
         B.rs = rs;
         blocks[o + b] = B;
+    }
+}
+
+// n_items = min(total of the key counts, capacity); raw total kept for the host.
+__global__ void k_items_total(const int64_t *key_off, int64_t nkeys, int64_t cap, int64_t *n_items,
+                              unsigned long long *raw) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const int64_t t = key_off[nkeys];
+        *n_items = t < cap ? t : cap;
+        *raw = (unsigned long long)t;
+    }
+}
+
+// Caller outputs from the accumulators: counts, DD (the pair count itself for
+// unit weights), S+D, their tallies, SxD, the variance and the region stats.
+__global__ void k_finalize(const FinSample *fs, const int64_t *fin0, int S) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= fin0[S]) return;
+    const int s = find_sample(fin0, S, i);
+    const FinSample &F = fs[s];
+    const int64_t j = i - fin0[s];
+    const int64_t nb = (int64_t)(1 + F.K) * F.B;
+    if (j < nb) {
+        const unsigned long long c = F.np[j];
+        F.out.npairs[j] = c;
+        if (F.out.dd) F.out.dd[j] = F.weighted ? F.dd[j] : (double)c;
+        F.out.splus[j] = F.sp[j];
+        if (j < F.B) {
+            F.out.scross[j] = F.sx[j];
+            if (F.out.variance) F.out.variance[j] = F.var_on ? F.var[j] : 0.0;
+        }
+        return;
+    }
+    const int64_t k = j - nb;  // region k < K
+    if (F.out.region_stats) {
+        double *r = F.out.region_stats + 4 * k;
+        r[0] = (double)F.rcnt[k];
+        r[1] = (double)F.rcnt[F.K + k];
+        r[2] = F.rsum[k];
+        r[3] = F.rsum[F.K + k];
     }
 }
 
@@ -350,9 +450,7 @@ __global__ void k_blocks_emit(const int32_t *sstart, Grid g, int ng, int G, int 
 struct PairArgs {
     const Item *items;
     const Block *blocks;
-    const int64_t *cost_prefix;  // exclusive, n_items + 1
-    int64_t n_items;
-    const int64_t *total_cost;   // device: cost prefix total (read by the kernel)
+    const int64_t *n_items;      // device: work items in the list
     int n_waves;
     const int64_t *bounds;       // schedule unit -> first item, n_waves * kScheduleUnits + 3
     int *queue;                  // tail-chunk counter, zero at launch
@@ -369,6 +467,8 @@ struct PairArgs {
     int copies_log2;             // each bin is kept in 2^copies_log2 interleaved copies
     int off_sp, off_sx, off_dd, off_var; // byte offsets inside a wave histogram
     unsigned long long *rechecks;
+    // wall-clock span of the launch's busy waves: [0] max of ~start, [1] max of end
+    unsigned long long *clock;
 };
 
 // Wave histogram layout (C = 2^copies_log2 interleaved copies of B bins;
@@ -704,7 +804,9 @@ __global__ __launch_bounds__(kThreads) void k_pairs(PairArgs P) {
     // runs of neighbouring items of the key-sorted list (which read the same
     // position cells) stay on one XCD
     const int w = xcd_run_block() * (blockDim.x / kWave) + wv;
-    if (w >= P.n_waves || P.n_items == 0) return;
+    const int64_t n_items = *(const MIA_CONST int64_t *)P.n_items;
+    if (w >= P.n_waves || n_items == 0) return;
+    if (lane == 0) atomicMax(&P.clock[0], ~(unsigned long long)wall_clock64());
     // a contiguous share of the cost-sorted items, then tail chunks from the counter
     const WaveSchedule sch = make_schedule(P.bounds, P.n_waves);
     int64_t it0, it1;
@@ -747,12 +849,11 @@ __global__ __launch_bounds__(kThreads) void k_pairs(PairArgs P) {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     if (Sp) flush_hist(P, hist, *Sp, cur_rs, cur_rp, lane);
     if (nre) atomicAdd(P.rechecks, (unsigned long long)nre);
+    if (lane == 0) atomicMax(&P.clock[1], (unsigned long long)wall_clock64());
 }
 
 template <int MODE, bool WEIGHTED, bool VAR>
 void launch_pairs(const PairArgs &A, unsigned grid, int wpb, size_t lds, hipStream_t st) {
-    MIA_HIP(hipFuncSetAttribute((const void *)k_pairs<MODE, WEIGHTED, VAR>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL((k_pairs<MODE, WEIGHTED, VAR>), dim3(grid), dim3(kWave * wpb), lds, st, A);
     MIA_HIP(hipGetLastError());
 }
@@ -786,224 +887,26 @@ size_t threshold_bytes(const mia_bins &bins) {
 
 inline unsigned nblk(int64_t n, int t) { return (unsigned)((n + t - 1) / t < 1 ? 1 : (n + t - 1) / t); }
 
-}  // namespace
+// Pair-kernel configuration of a binning: FP32 guess constants and guard
+// bands, passes over r-bin ranges, histogram copies and offsets, waves.
+struct PairConfig {
+    PairArgs A{};        // pointers unset
+    int nrp = 0, npass = 0, wpb = 0, waves = 0;
+    unsigned grid = 0;
+    size_t lds = 0;
+    double l0 = 0, lgs = 0;
+};
 
-bool fast_path_fits(const mia_bins &bins, bool weighted, int jk) {
-    const int64_t K1 = jk > 0 ? (int64_t)jk * jk * jk : 1;
-    if (K1 * K1 > kMaxKeysPerSample) return false;
-    // larger binnings run in passes over r-bin ranges: one r row must fit
-    mia_bins row = bins;
-    row.n_r = 1;
-    return threshold_bytes(bins) + wave_hist_bytes(row, weighted) <= kLdsBytes;
-}
-
-void count_fast(Ctx &ctx, std::vector<FastSample> &samples, const mia_bins &bins,
-                const mia_tuning *tuning, hipStream_t st, mia_stats *stats, const int *oor) {
-    const int ns_ = (int)samples.size();
+PairConfig pair_config(const mia_bins &bins, bool any_w, bool any_var, const mia_tuning *tuning, int cu_count) {
+    PairConfig C;
+    PairArgs &A = C.A;
     const int n_r = bins.n_r, n_2 = bins.n_2, B = n_r * n_2;
-    MIA_REQUIRE(n_r >= 1 && n_r <= kMaxBinsR && n_2 >= 1 && n_2 <= kMaxBins2, "bin counts out of range");
-    const double reach_r = std::sqrt(bins.r2_edges[n_r]);
-    const double reach_z = bins.mode == MIA_RPPI
-                               ? std::fmax(std::fabs(bins.x2_edges[0]), std::fabs(bins.x2_edges[n_2]))
-                               : reach_r;
-    const int nx2 = bins.mode == MIA_RPPI ? n_2 + 1 : n_2;
-    double *T = ctx.arena.get<double>("fast/T", n_r + 1);
-    double *X = ctx.arena.get<double>("fast/X", n_2 + 1);
-    std::vector<double> xh(n_2 + 1, 3.0);
-    for (int i = 0; i < nx2; ++i) xh[i] = bins.x2_edges[i];
-    MIA_HIP(hipMemcpyAsync(T, bins.r2_edges, sizeof(double) * (n_r + 1), hipMemcpyHostToDevice, st));
-    MIA_HIP(hipMemcpyAsync(X, xh.data(), sizeof(double) * (n_2 + 1), hipMemcpyHostToDevice, st));
-
-    Events ev(4);
-    MIA_HIP(hipEventRecord(ev[0], st));
-
-    // ---- plan: shape blocks per sample (one host sync for all samples)
-    const int bsz = tuning && tuning->block_shapes > 0 ? std::min(tuning->block_shapes, kMaxBlockShapes) : kMaxBlockShapes;
-    const int pmax = tuning && tuning->item_positions > 0 ? tuning->item_positions : 1024;
-    std::vector<int> blk_base(ns_ + 1, 0), gsz(ns_), ngr(ns_);
-    int32_t *btot = ctx.arena.get<int32_t>("fast/btot", 2 * (size_t)ns_);
-    int K1max = 1;
-    for (int si = 0; si < ns_; ++si) {
-        FastSample &F = samples[si];
-        const Grid &g = F.g;
-        K1max = std::max(K1max, g.jk > 0 ? g.jk * g.jk * g.jk : 1);
-        // slabs per block: explicit, else the grid's choice, else ~16 shapes
-        // per block; always a divisor of the per-jk slab count
-        int G = 1;
-        if (tuning && tuning->slabs_per_block > 0) {
-            G = tuning->slabs_per_block;
-        } else if (g.G > 0) {
-            G = g.G;
-        } else {
-            double per_slab = (double)F.shp.n / ((double)g.nc * g.nc * g.nz);
-            double want = per_slab > 0 ? 16.0 / per_slab : g.nzj;
-            G = (int)std::floor(want);
-        }
-        G = std::max(1, std::min(G, g.nzj));
-        while (g.nzj % G) --G;
-        gsz[si] = G;
-        ngr[si] = g.nz / G;
-        int ncol = g.nc * g.nc;
-        int64_t nt = (int64_t)ncol * (ngr[si] + 1);
-        int32_t *nb = ctx.arena.get<int32_t>("fast/nblk" + std::to_string(si), nt + 1);
-        int32_t *nbo = ctx.arena.get<int32_t>("fast/nblko" + std::to_string(si), nt + 1);
-        hipLaunchKernelGGL(k_blocks_count, dim3(nblk(nt, 256)), dim3(256), 0, st, F.shp.start, g,
-                           ngr[si], G, bsz, nb);
-        MIA_HIP(hipGetLastError());
-        exclusive_scan<int32_t>(ctx, nb, nbo, nt, st, "fast/bs" + std::to_string(si), 0);
-        MIA_HIP(hipMemcpyAsync(btot + si, nbo + nt, sizeof(int32_t), hipMemcpyDeviceToDevice, st));
-    }
-    {
-        // one readback: block totals and the samples' out-of-box flags
-        std::vector<int32_t> h(2 * ns_, 0);
-        if (oor)
-            MIA_HIP(hipMemcpyAsync(btot + ns_, oor, sizeof(int32_t) * ns_, hipMemcpyDeviceToDevice, st));
-        MIA_HIP(hipMemcpyAsync(h.data(), btot, sizeof(int32_t) * (oor ? 2 * ns_ : ns_),
-                               hipMemcpyDeviceToHost, st));
-        MIA_HIP(hipStreamSynchronize(st));
-        for (int si = 0; si < ns_; ++si)
-            if (h[ns_ + si])
-                throw Error{MIA_E_RANGE, "coordinates outside [0, boxsize] (or NaN) in sample " +
-                                             std::to_string(si)};
-        for (int si = 0; si < ns_; ++si) blk_base[si + 1] = blk_base[si] + h[si];
-    }
-    const int nblocks = blk_base[ns_];
-    Block *blocks = ctx.arena.get<Block>("fast/blocks", (size_t)std::max(nblocks, 1));
-    for (int si = 0; si < ns_; ++si) {
-        FastSample &F = samples[si];
-        const Grid &g = F.g;
-        int64_t nt = (int64_t)g.nc * g.nc * (ngr[si] + 1);
-        int32_t *nbo = ctx.arena.get<int32_t>("fast/nblko" + std::to_string(si), nt + 1);
-        hipLaunchKernelGGL(k_blocks_emit, dim3(nblk(nt, 256)), dim3(256), 0, st, F.shp.start, g,
-                           ngr[si], gsz[si], bsz, nbo, si, blocks + blk_base[si]);
-        MIA_HIP(hipGetLastError());
-    }
-
-    // ---- plan: work items, counting-sorted by (sample, R_s, R_p)
-    const int K1 = K1max;
-    const int64_t nkeys = (int64_t)ns_ * K1 * K1;
-    const size_t plan_lds = (size_t)K1 * K1 * (sizeof(int64_t) + sizeof(int32_t));
-    const int plan_local = plan_lds <= 64 * 1024 ? 1 : 0;
-    unsigned long long *kc = ctx.arena.get<unsigned long long>("fast/keycount", nkeys + 1);
-    int64_t *ko = ctx.arena.get<int64_t>("fast/keyoff", nkeys + 1);
-    unsigned long long *kcur = ctx.arena.get<unsigned long long>("fast/keycur", nkeys + 1);
-    fill<unsigned long long>(kc, 0ull, nkeys + 1, st);
-    fill<unsigned long long>(kcur, 0ull, nkeys + 1, st);
-    std::vector<PlanArgs> plans(ns_);
-    for (int si = 0; si < ns_; ++si) {
-        FastSample &F = samples[si];
-        PlanArgs &P = plans[si];
-        P.g = F.g;
-        P.mode = bins.mode;
-        P.periodic = F.s->periodic;
-        P.sample = si;
-        P.K1 = K1;
-        P.ng = ngr[si];
-        P.G = gsz[si];
-        P.kx = (int)std::ceil((reach_r + 4 * F.g.margin) / F.g.cw) + 1;
-        P.nxw = std::min(F.g.nc, 2 * P.kx + 1);
-        P.pmax = pmax;
-        P.reach_r = reach_r;
-        P.reach_z = reach_z;
-        P.sstart = F.shp.start;
-        P.pstart = F.pos.start;
-        P.blocks = blocks + blk_base[si];
-        P.nblocks = blk_base[si + 1] - blk_base[si];
-        P.blk_base = blk_base[si];
-        P.key_count = kc;
-        P.key_cursor = kcur;
-        P.key_off = ko;
-        P.items = nullptr;
-        P.cost = nullptr;
-        P.candidates = nullptr;
-        P.emit = 0;
-        P.local = plan_local;
-        int64_t nt = (int64_t)P.nblocks * P.nxw * P.nxw;
-        if (nt > 0) {
-            hipLaunchKernelGGL(k_plan_items, dim3(nblk(nt, kPlanThreads)), dim3(kPlanThreads),
-                               plan_local ? plan_lds : 0, st, P);
-            MIA_HIP(hipGetLastError());
-        }
-    }
-    exclusive_scan<int64_t>(ctx, (const int64_t *)kc, ko, nkeys, st, "fast/keys", 0);
-    int64_t nitems = 0;
-    MIA_HIP(hipMemcpyAsync(&nitems, ko + nkeys, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-    MIA_HIP(hipStreamSynchronize(st));
-    Item *items = ctx.arena.get<Item>("fast/items", (size_t)std::max<int64_t>(nitems, 1));
-    int64_t *cost = ctx.arena.get<int64_t>("fast/cost", nitems + 1);
-    int64_t *cpre = ctx.arena.get<int64_t>("fast/costpre", nitems + 1);
-    // [0] rechecks, [1] candidates: read back together after the pair kernel
-    unsigned long long *counters = ctx.arena.get<unsigned long long>("fast/counters", 2);
-    fill<unsigned long long>(counters, 0ull, 2, st);
-    unsigned long long *ncand = counters + 1;
-    // one tail-chunk counter per r-bin pass (n_r <= kMaxBinsR)
-    int32_t *queue = ctx.arena.get<int32_t>("fast/queue", kMaxBinsR);
-    fill<int32_t>(queue, 0, kMaxBinsR, st);
-    for (int si = 0; si < ns_; ++si) {
-        PlanArgs P = plans[si];
-        P.emit = 1;
-        P.items = items;
-        P.cost = cost;
-        P.candidates = ncand;
-        int64_t nt = (int64_t)P.nblocks * P.nxw * P.nxw;
-        if (nt > 0) {
-            hipLaunchKernelGGL(k_plan_items, dim3(nblk(nt, kPlanThreads)), dim3(kPlanThreads),
-                               plan_local ? plan_lds : 0, st, P);
-            MIA_HIP(hipGetLastError());
-        }
-    }
-    exclusive_scan<int64_t>(ctx, cost, cpre, nitems, st, "fast/cost", 0);
-    MIA_HIP(hipEventRecord(ev[1], st));
-
-    // ---- accumulators
-    const bool any_var = bins.accumulate_variance != 0;
-    bool any_w = false;
-    std::vector<SampleDev> sd(ns_);
-    for (int si = 0; si < ns_; ++si) {
-        FastSample &F = samples[si];
-        any_w = any_w || F.weighted;
-        SampleDev &S = sd[si];
-        std::memset(&S, 0, sizeof(S));
-        S.pa = F.pos.a; S.pb = F.pos.b; S.pl = F.pos.l; S.pw = F.pos.w;
-        S.srec = F.shp.rec;
-        S.pstart = F.pos.start;
-        S.L = F.g.L;
-        S.half = F.g.half;
-        S.los = F.g.los;
-        S.K = F.g.jk > 0 ? F.g.jk * F.g.jk * F.g.jk : 0;
-        S.nbins = B;
-        int64_t nb = (int64_t)(1 + S.K) * B;
-        // the kernel accumulates straight into the caller's outputs
-        S.npairs = (unsigned long long *)F.out->npairs;
-        S.sp = F.out->splus;
-        S.sx = F.out->scross;
-        S.dd = F.weighted ? F.out->dd : nullptr;
-        S.var = any_var ? F.out->variance : nullptr;
-        fill<unsigned long long>(S.npairs, 0ull, nb, st);
-        fill<double>(S.sp, 0.0, nb, st);
-        fill<double>(S.sx, 0.0, B, st);
-        if (S.dd) fill<double>(S.dd, 0.0, nb, st);
-        if (F.out->variance) fill<double>(F.out->variance, 0.0, B, st);
-    }
-    SampleDev *sdd = ctx.arena.get<SampleDev>("fast/samples", ns_);
-    MIA_HIP(hipMemcpyAsync(sdd, sd.data(), sizeof(SampleDev) * ns_, hipMemcpyHostToDevice, st));
-    unsigned long long *rechecks = counters;
-
-    // ---- pair kernel
-    PairArgs A{};
-    A.items = items;
-    A.blocks = blocks;
-    A.cost_prefix = cpre;
-    A.n_items = nitems;
-    A.total_cost = cpre + nitems;
     A.mode = bins.mode;
     A.n_r = n_r;
     A.n_2 = n_2;
     A.weighted = any_w ? 1 : 0;
     A.var = any_var ? 1 : 0;
     A.nbins = B;
-    A.T = T;
-    A.X = X;
     // FP32 bin guesses and their guard bands (bin units).  The guess error
     // is bounded by a few FP32 ulps of the log2 / linear transform (the
     // bounds below, with a 20x margin); the exact FP64 count runs only for
@@ -1012,6 +915,8 @@ void count_fast(Ctx &ctx, std::vector<FastSample> &samples, const mia_bins &bins
     A.Tn = bins.r2_edges[n_r];
     double l0 = std::log2(bins.r2_edges[0]), ln = std::log2(bins.r2_edges[n_r]);
     double lgs = n_r / (ln - l0);
+    C.l0 = l0;
+    C.lgs = lgs;
     A.lgs = (float)lgs;
     A.lgc = (float)(-l0 * lgs);
     const double u = std::ldexp(1.0, -23);  // FP32 ulp of 1
@@ -1039,7 +944,6 @@ void count_fast(Ctx &ctx, std::vector<FastSample> &samples, const mia_bins &bins
     A.dr = exact_only ? 1.0f : (float)dr;
     A.d2 = exact_only ? 1.0f : (float)d2;
     A.scan = exact_only ? 1 : 0;
-    A.samples = sdd;
     // Passes over r-bin ranges.  Pass [r0, r0 + nrp) takes the pairs with
     // T[r0] <= r^2 < T[r0 + nrp] (exact FP64 comparisons, so every pair lands
     // in one pass), bins them locally and flushes to output bins
@@ -1086,7 +990,8 @@ void count_fast(Ctx &ctx, std::vector<FastSample> &samples, const mia_bins &bins
             }
         }
     }
-    const int npass = (n_r + nrp - 1) / nrp;
+    C.nrp = nrp;
+    C.npass = (n_r + nrp - 1) / nrp;
     const int Bp = nrp * n_2;     // bins of the largest pass
     A.nbins_out = B;
     A.bin_off = 0;
@@ -1114,55 +1019,404 @@ void count_fast(Ctx &ctx, std::vector<FastSample> &samples, const mia_bins &bins
     A.off_var = off;
     if (A.var) off += 8 * BC;
     A.hist_stride = (off + 15) & ~15;
-    A.rechecks = rechecks;
     // waves per block: 4, or fewer when the histograms of 4 waves do not fit
     int wpb = kMaxWavesPerBlock;
     while (wpb > 1 && thr_bytes + (size_t)A.hist_stride * wpb > kLdsBytes) wpb /= 2;
-    const size_t lds = thr_bytes + (size_t)A.hist_stride * wpb;
-    if (lds > kLdsBytes)
+    C.lds = thr_bytes + (size_t)A.hist_stride * wpb;
+    if (C.lds > kLdsBytes)
         throw Error{MIA_E_UNSUPPORTED, "histogram too large for LDS (n_r*n_2 too big for MIA_FAST)"};
+    C.wpb = wpb;
     // at most 8 waves per SIMD (32 per CU)
-    int blocks_per_cu = (int)std::max<size_t>(1, std::min<size_t>(32 / wpb, kLdsBytes / lds));
-    int waves = tuning && tuning->waves > 0 ? std::min(tuning->waves, kMaxWaves) : ctx.cu_count * blocks_per_cu * wpb;
-    if (nitems < waves) waves = (int)std::max<int64_t>(1, nitems);
-    A.n_waves = waves;
-    unsigned grid = (unsigned)((waves + wpb - 1) / wpb);
-    int64_t *bounds = ctx.arena.get<int64_t>("fast/bounds", (size_t)waves * kScheduleUnits + 3);
-    if (nitems > 0) schedule_bounds(cpre, nitems, waves, bounds, st);
-    A.bounds = bounds;
-    for (int pass = 0; pass < npass && nitems > 0; ++pass) {
-        const int r0 = pass * nrp, r1 = std::min(n_r, r0 + nrp);
+    int blocks_per_cu = (int)std::max<size_t>(1, std::min<size_t>(32 / wpb, kLdsBytes / C.lds));
+    int waves = tuning && tuning->waves > 0 ? std::min(tuning->waves, kMaxWaves) : cu_count * blocks_per_cu * wpb;
+    C.waves = std::max(1, waves);
+    A.n_waves = C.waves;
+    C.grid = (unsigned)((C.waves + wpb - 1) / wpb);
+    return C;
+}
+
+}  // namespace
+
+bool fast_path_fits(const mia_bins &bins, bool weighted, int jk) {
+    const int64_t K1 = jk > 0 ? (int64_t)jk * jk * jk : 1;
+    if (K1 * K1 > kMaxKeysPerSample) return false;
+    // larger binnings run in passes over r-bin ranges: one r row must fit
+    mia_bins row = bins;
+    row.n_r = 1;
+    return threshold_bytes(bins) + wave_hist_bytes(row, weighted) <= kLdsBytes;
+}
+
+// Sets the dynamic-LDS limit of every pair-kernel variant (once per process).
+void pair_kernels_init() {
+    static std::mutex mu;
+    static uint64_t done = 0;  // one bit per device
+    int dev = 0;
+    MIA_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(mu);
+    if (dev < 64 && (done >> dev & 1)) return;
+    const void *fns[] = {
+        (const void *)k_pairs<MIA_RPPI, false, false>, (const void *)k_pairs<MIA_RPPI, false, true>,
+        (const void *)k_pairs<MIA_RPPI, true, false>,  (const void *)k_pairs<MIA_RPPI, true, true>,
+        (const void *)k_pairs<MIA_RMU, false, false>,  (const void *)k_pairs<MIA_RMU, false, true>,
+        (const void *)k_pairs<MIA_RMU, true, false>,   (const void *)k_pairs<MIA_RMU, true, true>};
+    for (const void *f : fns)
+        MIA_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBytes));
+    if (dev < 64) done |= 1ull << dev;
+}
+
+// ---------------------------------------------------------------------------
+// staged count: host plan, workspace binding, launch sequence
+
+struct CountPlan {
+    int S = 0;
+    int K1 = 1, plan_local = 0, bsz = kMaxBlockShapes;
+    int64_t ncells = 0, nthreads = 0, nkeys = 0, fin_total = 0, item_cap = 0;
+    int64_t acc_total = 0;           // accumulator words (u64 / f64) of all samples
+    PairConfig pc;
+    bool any_w = false, any_var = false;
+    std::vector<double> T, X;        // thresholds (X padded to n_2 + 1)
+    std::vector<PlanSample> ps;
+    std::vector<SampleDev> sd;
+    std::vector<FinSample> fs;
+    std::vector<int64_t> cell0, t0, fin0;
+    std::vector<int64_t> acc0;       // first accumulator word of each sample
+    std::vector<int> Bs;             // per sample: K
+    // bound workspace
+    char *param_dev = nullptr;
+    std::vector<char> param;         // host image of the parameter block
+    size_t off_T = 0, off_X = 0, off_cell0 = 0, off_t0 = 0, off_fin0 = 0, off_ps = 0, off_sd = 0, off_fs = 0;
+    size_t param_bytes = 0;
+    unsigned long long *zero_begin = nullptr;
+    size_t zero_words = 0;
+    int32_t *nb = nullptr, *boff = nullptr;
+    Block *blocks = nullptr;
+    unsigned long long *kc = nullptr, *kcur = nullptr;
+    int64_t *ko = nullptr;
+    Item *items = nullptr;
+    int64_t *cost = nullptr, *cpre = nullptr, *n_items = nullptr, *bounds = nullptr;
+    unsigned long long *words = nullptr;   // [kCountReadback] read back after the count
+    int *queue = nullptr;
+    unsigned long long *acc = nullptr;
+    ScanBufs<int32_t> scan_nb;
+    ScanBufs<int64_t> scan_keys, scan_cost;
+    std::vector<int> G;
+    std::string key;
+};
+
+int count_waves(const CountPlan &p) { return p.pc.waves; }
+int count_bsz(const CountPlan &p) { return p.bsz; }
+int count_G(const CountPlan &p, int s) { return p.G[s]; }
+int64_t count_item_cap(const CountPlan &p) { return p.item_cap; }
+std::string count_key(const CountPlan &p) { return p.key; }
+size_t count_param_bytes(const CountPlan &p) { return p.param_bytes; }
+const void *count_param_image(const CountPlan &p) { return p.param.data(); }
+void *count_param_dev(const CountPlan &p) { return p.param_dev; }
+const unsigned long long *count_readback(const CountPlan &p) { return p.words; }
+void free_plan(CountPlan *p) { delete p; }
+
+// Expected work items of a sample (mia_grid.hip's uniform-density model).
+double estimate_items(const Grid &g, int G, double reach_r, double reach_z, bool rmu, int64_t ns, int64_t np);
+
+CountPlan *plan_count(Ctx &ctx, const std::vector<SortedSample> &ss, const mia_bins &bins,
+                      const mia_tuning *tuning, int64_t item_cap) {
+    pair_kernels_init();
+    std::unique_ptr<CountPlan> P(new CountPlan());
+    CountPlan &p = *P;
+    const int S = (int)ss.size();
+    MIA_REQUIRE(S >= 1, "no samples");
+    p.S = S;
+    const int n_r = bins.n_r, n_2 = bins.n_2, B = n_r * n_2;
+    MIA_REQUIRE(n_r >= 1 && n_r <= kMaxBinsR && n_2 >= 1 && n_2 <= kMaxBins2, "bin counts out of range");
+    const double reach_r = std::sqrt(bins.r2_edges[n_r]);
+    const double reach_z = bins.mode == MIA_RPPI
+                               ? std::fmax(std::fabs(bins.x2_edges[0]), std::fabs(bins.x2_edges[n_2]))
+                               : reach_r;
+    const int nx2 = bins.mode == MIA_RPPI ? n_2 + 1 : n_2;
+    p.T.assign(bins.r2_edges, bins.r2_edges + n_r + 1);
+    p.X.assign(n_2 + 1, 3.0);
+    for (int i = 0; i < nx2; ++i) p.X[i] = bins.x2_edges[i];
+    p.bsz = tuning && tuning->block_shapes > 0 ? std::min(tuning->block_shapes, kMaxBlockShapes) : kMaxBlockShapes;
+    const int pmax = tuning && tuning->item_positions > 0 ? tuning->item_positions : 1024;
+    p.any_var = bins.accumulate_variance != 0;
+    for (const auto &x : ss) p.any_w = p.any_w || x.weighted;
+    p.pc = pair_config(bins, p.any_w, p.any_var, tuning, ctx.cu_count);
+
+    p.ps.resize(S);
+    p.sd.resize(S);
+    p.fs.resize(S);
+    p.G.resize(S);
+    p.cell0.assign(S + 1, 0);
+    p.t0.assign(S + 1, 0);
+    p.fin0.assign(S + 1, 0);
+    p.acc0.assign(S + 1, 0);
+    double est_items = 0;
+    for (int si = 0; si < S; ++si) {
+        const SortedSample &x = ss[si];
+        const Grid &g = x.g;
+        p.K1 = std::max(p.K1, x.K > 0 ? x.K : 1);
+        // slabs per block: explicit, else the grid's choice, else ~16 shapes
+        // per block; always a divisor of the per-jk slab count
+        int G = 1;
+        if (tuning && tuning->slabs_per_block > 0) {
+            G = tuning->slabs_per_block;
+        } else if (g.G > 0) {
+            G = g.G;
+        } else {
+            double per_slab = (double)x.n_shape / ((double)g.nc * g.nc * g.nz);
+            double want = per_slab > 0 ? 16.0 / per_slab : g.nzj;
+            G = (int)std::floor(want);
+        }
+        G = std::max(1, std::min(G, g.nzj));
+        while (g.nzj % G) --G;
+        p.G[si] = G;
+        PlanSample &Q = p.ps[si];
+        std::memset(&Q, 0, sizeof(Q));
+        Q.g = g;
+        Q.mode = bins.mode;
+        Q.periodic = x.periodic;
+        Q.ng = g.nz / G;
+        Q.G = G;
+        Q.kx = (int)std::ceil((reach_r + 4 * g.margin) / g.cw) + 1;
+        Q.nxw = std::min(g.nc, 2 * Q.kx + 1);
+        Q.pmax = pmax;
+        Q.reach_r = reach_r;
+        Q.reach_z = reach_z;
+        Q.sstart = x.sstart;
+        Q.pstart = x.pstart;
+        Q.ncells = (int64_t)g.nc * g.nc * (Q.ng + 1);
+        Q.cell0 = p.cell0[si];
+        p.cell0[si + 1] = Q.cell0 + Q.ncells;
+        const int64_t blk_cap = (x.n_shape + p.bsz - 1) / p.bsz + std::min<int64_t>(Q.ncells, x.n_shape);
+        Q.nt = blk_cap * Q.nxw * Q.nxw;
+        Q.t0 = p.t0[si];
+        p.t0[si + 1] = Q.t0 + (Q.nt + kPlanThreads - 1) / kPlanThreads * kPlanThreads;
+        const int K = x.K;
+        p.fin0[si + 1] = p.fin0[si] + (int64_t)(1 + K) * B + K;
+        // accumulators: npairs, sp, dd ((1+K) B each), sx, var (B each)
+        p.acc0[si + 1] = p.acc0[si] + 3 * (int64_t)(1 + K) * B + 2 * (int64_t)B;
+        est_items += estimate_items(g, G, reach_r, reach_z, bins.mode == MIA_RMU, x.n_shape, x.n_pos);
+    }
+    p.ncells = p.cell0[S];
+    p.nthreads = p.t0[S];
+    p.fin_total = p.fin0[S];
+    p.acc_total = p.acc0[S];
+    MIA_REQUIRE((double)S * p.K1 * p.K1 < 2e9, "too many planner keys (samples x regions^2)");
+    p.nkeys = (int64_t)S * p.K1 * p.K1;
+    const size_t plan_lds = (size_t)p.K1 * p.K1 * (sizeof(int64_t) + sizeof(int32_t));
+    p.plan_local = plan_lds <= 64 * 1024 ? 1 : 0;
+    // item capacity: the caller's (grown after an overflow), at least 3x the
+    // uniform-density estimate plus a margin
+    p.item_cap = std::max<int64_t>(item_cap, (int64_t)(3.0 * est_items) + 65536);
+    p.item_cap = std::min<int64_t>(p.item_cap, (int64_t)1 << 31);
+
+    // the launch-sequence key: everything the launches depend on except the
+    // caller's pointers (which live in the parameter block)
+    std::string &k = p.key;
+    auto put = [&](const void *v, size_t n) { k.append((const char *)v, n); };
+    put(&S, sizeof(S));
+    put(&bins.mode, sizeof(int) * 4);
+    put(p.T.data(), sizeof(double) * p.T.size());
+    put(p.X.data(), sizeof(double) * p.X.size());
+    if (tuning) put(tuning, sizeof(*tuning));
+    put(&p.item_cap, sizeof(p.item_cap));
+    for (int si = 0; si < S; ++si) {
+        PlanSample q = p.ps[si];
+        q.sstart = q.pstart = nullptr;
+        put(&q, sizeof(q));
+        put(&ss[si].n_pos, sizeof(int64_t));
+        put(&ss[si].n_shape, sizeof(int64_t));
+        put(&ss[si].K, sizeof(int));
+        put(&ss[si].weighted, sizeof(int));
+    }
+    return P.release();
+}
+
+size_t count_bytes(const CountPlan &p) {
+    Carve c;
+    CountPlan q = p;
+    count_bind(q, c, {}, nullptr);
+    return c.off;
+}
+
+void count_bind(CountPlan &p, Carve &c, const std::vector<SortedSample> &ss, const mia_counts *outs) {
+    const int S = p.S;
+    const int n_r = p.pc.A.n_r, n_2 = p.pc.A.n_2, B = n_r * n_2;
+    const bool dry = c.dry;
+    // parameter block
+    Carve pc;
+    pc.dry = true;
+    p.off_T = pc.off; pc.take<double>(n_r + 1);
+    p.off_X = pc.off; pc.take<double>(n_2 + 1);
+    p.off_cell0 = pc.off; pc.take<int64_t>(S + 1);
+    p.off_t0 = pc.off; pc.take<int64_t>(S + 1);
+    p.off_fin0 = pc.off; pc.take<int64_t>(S + 1);
+    p.off_ps = pc.off; pc.take<PlanSample>(S);
+    p.off_sd = pc.off; pc.take<SampleDev>(S);
+    p.off_fs = pc.off; pc.take<FinSample>(S);
+    p.param_bytes = pc.off;
+    p.param_dev = c.take<char>(p.param_bytes);
+    // zeroed at the start of every count: accumulators, key counters,
+    // read-back words, tail-chunk counters (one contiguous range)
+    char *z0 = c.dry ? nullptr : c.base + c.off;
+    p.acc = c.take<unsigned long long>(p.acc_total);
+    p.kc = c.take<unsigned long long>(p.nkeys + 1);
+    p.kcur = c.take<unsigned long long>(p.nkeys + 1);
+    p.words = c.take<unsigned long long>(kCountReadback);
+    p.queue = c.take<int>(kMaxBinsR);
+    p.zero_begin = (unsigned long long *)z0;
+    p.zero_words = dry ? 0 : (size_t)((c.base + c.off) - z0) / 8;
+    // plan
+    p.nb = c.take<int32_t>(p.ncells + 1);
+    p.boff = c.take<int32_t>(p.ncells + 1);
+    scan_carve<int32_t>(c, p.ncells, p.scan_nb);
+    int64_t blk_total = 0;
+    for (int si = 0; si < S; ++si) blk_total += p.ps[si].nt / (p.ps[si].nxw * p.ps[si].nxw);
+    p.blocks = c.take<Block>(std::max<int64_t>(blk_total, 1));
+    p.ko = c.take<int64_t>(p.nkeys + 1);
+    scan_carve<int64_t>(c, p.nkeys, p.scan_keys);
+    p.items = c.take<Item>(p.item_cap);
+    p.cost = c.take<int64_t>(p.item_cap + 1);
+    p.cpre = c.take<int64_t>(p.item_cap + 1);
+    scan_carve<int64_t>(c, p.item_cap, p.scan_cost);
+    p.n_items = c.take<int64_t>(1);
+    p.bounds = c.take<int64_t>((size_t)p.pc.waves * kScheduleUnits + 3);
+    if (dry) return;
+
+    // parameter block image
+    p.param.assign(p.param_bytes, 0);
+    char *img = p.param.data();
+    std::memcpy(img + p.off_T, p.T.data(), sizeof(double) * (n_r + 1));
+    std::memcpy(img + p.off_X, p.X.data(), sizeof(double) * (n_2 + 1));
+    std::memcpy(img + p.off_cell0, p.cell0.data(), sizeof(int64_t) * (S + 1));
+    std::memcpy(img + p.off_t0, p.t0.data(), sizeof(int64_t) * (S + 1));
+    std::memcpy(img + p.off_fin0, p.fin0.data(), sizeof(int64_t) * (S + 1));
+    unsigned long long *acc = p.acc;
+    double *accd = (double *)p.acc;
+    for (int si = 0; si < S; ++si) {
+        const SortedSample &x = ss[si];
+        const int K = x.K;
+        const int64_t nb = (int64_t)(1 + K) * B;
+        const int64_t a0 = p.acc0[si];
+        PlanSample &Q = p.ps[si];
+        Q.sstart = x.sstart;
+        Q.pstart = x.pstart;
+        SampleDev &D = p.sd[si];
+        std::memset(&D, 0, sizeof(D));
+        D.pa = x.pa; D.pb = x.pb; D.pl = x.pl; D.pw = x.pw;
+        D.srec = x.srec;
+        D.pstart = x.pstart;
+        D.L = x.g.L;
+        D.half = x.g.half;
+        D.los = x.g.los;
+        D.K = K;
+        D.nbins = B;
+        D.npairs = acc + a0;
+        D.sp = accd + a0 + nb;
+        D.dd = x.weighted ? accd + a0 + 2 * nb : nullptr;
+        D.sx = accd + a0 + 3 * nb;
+        D.var = p.any_var ? accd + a0 + 3 * nb + B : nullptr;
+        FinSample &F = p.fs[si];
+        std::memset(&F, 0, sizeof(F));
+        F.np = acc + a0;
+        F.sp = accd + a0 + nb;
+        F.dd = accd + a0 + 2 * nb;
+        F.sx = accd + a0 + 3 * nb;
+        F.var = accd + a0 + 3 * nb + B;
+        F.rcnt = x.rcnt;
+        F.rsum = x.rsum;
+        F.out = outs[si];
+        F.K = K;
+        F.B = B;
+        F.weighted = x.weighted;
+        F.var_on = p.any_var ? 1 : 0;
+    }
+    std::memcpy(img + p.off_ps, p.ps.data(), sizeof(PlanSample) * S);
+    std::memcpy(img + p.off_sd, p.sd.data(), sizeof(SampleDev) * S);
+    std::memcpy(img + p.off_fs, p.fs.data(), sizeof(FinSample) * S);
+}
+
+void count_enqueue(const CountPlan &p, hipStream_t st, const hipEvent_t *ev) {
+    const int S = p.S;
+    const char *pd = p.param_dev;
+    const PlanSample *ps = (const PlanSample *)(pd + p.off_ps);
+    const int64_t *cell0 = (const int64_t *)(pd + p.off_cell0);
+    const int64_t *t0 = (const int64_t *)(pd + p.off_t0);
+    const int64_t *fin0 = (const int64_t *)(pd + p.off_fin0);
+    if (ev) MIA_HIP(hipEventRecord(ev[0], st));
+    fill<unsigned long long>(p.zero_begin, 0ull, (int64_t)p.zero_words, st);
+    // shape blocks of every (column, slab group) cell
+    hipLaunchKernelGGL(k_blocks, dim3(nblk(p.ncells, 256)), dim3(256), 0, st, ps, cell0, S, p.bsz, p.nb,
+                       (const int32_t *)nullptr, (Block *)nullptr);
+    MIA_HIP(hipGetLastError());
+    scan_ws<int32_t>(p.nb, p.boff, p.ncells, nullptr, p.scan_nb, st);
+    hipLaunchKernelGGL(k_blocks, dim3(nblk(p.ncells, 256)), dim3(256), 0, st, ps, cell0, S, p.bsz, p.nb,
+                       (const int32_t *)p.boff, p.blocks);
+    MIA_HIP(hipGetLastError());
+    // work items, counting-sorted by (sample, R_s, R_p)
+    PlanLaunch L{};
+    L.ps = ps;
+    L.t0 = t0;
+    L.S = S;
+    L.K1 = p.K1;
+    L.boff = p.boff;
+    L.blocks = p.blocks;
+    L.key_count = p.kc;
+    L.key_cursor = p.kcur;
+    L.key_off = p.ko;
+    L.items = p.items;
+    L.cost = p.cost;
+    L.cap = p.item_cap;
+    L.overflow = p.words + 2;
+    L.candidates = p.words + 1;
+    L.local = p.plan_local;
+    const size_t plan_lds = p.plan_local ? (size_t)p.K1 * p.K1 * 12 : 0;
+    const int64_t nvwg = (p.nthreads + kPlanThreads - 1) / kPlanThreads;
+    const unsigned pgrid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(nvwg, 256 * kPlanGridPerCu));
+    L.emit = 0;
+    hipLaunchKernelGGL(k_plan_items, dim3(pgrid), dim3(kPlanThreads), plan_lds, st, L);
+    MIA_HIP(hipGetLastError());
+    scan_ws<int64_t>((const int64_t *)p.kc, p.ko, p.nkeys, nullptr, p.scan_keys, st);
+    hipLaunchKernelGGL(k_items_total, dim3(1), dim3(64), 0, st, (const int64_t *)p.ko, p.nkeys, p.item_cap,
+                       p.n_items, p.words + 3);
+    MIA_HIP(hipGetLastError());
+    L.emit = 1;
+    hipLaunchKernelGGL(k_plan_items, dim3(pgrid), dim3(kPlanThreads), plan_lds, st, L);
+    MIA_HIP(hipGetLastError());
+    scan_ws<int64_t>(p.cost, p.cpre, p.item_cap, p.n_items, p.scan_cost, st);
+    schedule_bounds(p.cpre, p.n_items, p.pc.waves, p.bounds, st);
+    if (ev) MIA_HIP(hipEventRecord(ev[1], st));
+    // pair kernel, one launch per r-bin pass
+    PairArgs A = p.pc.A;
+    A.items = p.items;
+    A.blocks = p.blocks;
+    A.n_items = p.n_items;
+    A.bounds = p.bounds;
+    A.T = (const double *)(pd + p.off_T);
+    A.X = (const double *)(pd + p.off_X);
+    A.samples = (const SampleDev *)(pd + p.off_sd);
+    A.rechecks = p.words;
+    A.clock = p.words + 4;
+    const int n_r = p.pc.A.n_r, n_2 = p.pc.A.n_2;
+    for (int pass = 0; pass < p.pc.npass; ++pass) {
+        const int r0 = pass * p.pc.nrp, r1 = std::min(n_r, r0 + p.pc.nrp);
         A.n_r = r1 - r0;
         A.nbins = (r1 - r0) * n_2;
         A.bin_off = r0 * n_2;
-        A.T = T + r0;
-        A.T0 = bins.r2_edges[r0];
-        A.Tn = bins.r2_edges[r1];
-        A.lgc = (float)(-l0 * lgs - r0);
-        A.queue = queue + pass;
-        if (bins.mode == MIA_RPPI)
-            launch_pairs_mode<MIA_RPPI>(A, grid, wpb, lds, st);
+        A.T = (const double *)(pd + p.off_T) + r0;
+        A.T0 = p.T[r0];
+        A.Tn = p.T[r1];
+        A.lgc = (float)(-p.pc.l0 * p.pc.lgs - r0);
+        A.queue = p.queue + pass;
+        if (A.mode == MIA_RPPI)
+            launch_pairs_mode<MIA_RPPI>(A, p.pc.grid, p.pc.wpb, p.pc.lds, st);
         else
-            launch_pairs_mode<MIA_RMU>(A, grid, wpb, lds, st);
+            launch_pairs_mode<MIA_RMU>(A, p.pc.grid, p.pc.wpb, p.pc.lds, st);
     }
-    MIA_HIP(hipEventRecord(ev[2], st));
-    MIA_HIP(hipEventRecord(ev[3], st));
-    MIA_HIP(hipEventSynchronize(ev[3]));
-    const float t01 = ev.ms(0, 1), t12 = ev.ms(1, 2), t23 = ev.ms(2, 3);
-    unsigned long long hc[2] = {0, 0};
-    MIA_HIP(hipMemcpy(hc, counters, sizeof(hc), hipMemcpyDeviceToHost));
-    const unsigned long long nre = hc[0], ncand_h = hc[1];
-    if (stats) {
-        stats->ms_plan = t01;
-        stats->ms_pairs = t12;
-        stats->ms_finalize = t23;
-        stats->candidates = (int64_t)ncand_h;
-        stats->items = nitems;
-        stats->rechecks = (int64_t)nre;
-        stats->slabs_per_block = gsz.empty() ? 0 : gsz[0];
-        stats->block_shapes = bsz;
-        stats->waves = waves;
-    }
+    if (ev) MIA_HIP(hipEventRecord(ev[2], st));
+    hipLaunchKernelGGL(k_finalize, dim3(nblk(p.fin_total, 256)), dim3(256), 0, st,
+                       (const FinSample *)(pd + p.off_fs), fin0, S);
+    MIA_HIP(hipGetLastError());
+    if (ev) MIA_HIP(hipEventRecord(ev[3], st));
 }
 
 }  // namespace mia

@@ -115,6 +115,14 @@ Choice search(double L, int jk, double R, double Z, bool rmu, int64_t ns, int64_
 
 }  // namespace
 
+double estimate_items(const Grid &g, int G, double reach_r, double reach_z, bool rmu, int64_t ns, int64_t np) {
+    const int jk = g.jk > 0 ? g.jk : 1;
+    double cand = 0, items = 0;
+    estimate(g.L, jk, g.ncj, g.nzj, std::max(1, G), reach_r, reach_z, rmu, (double)std::max<int64_t>(ns, 1),
+             (double)std::max<int64_t>(np, 1), cand, items);
+    return ns > 0 && np > 0 ? items : 0.0;
+}
+
 Grid make_grid(const mia_sample &s, const mia_bins &bins, const mia_tuning *t) {
     Grid g;
     g.L = s.boxsize;

@@ -30,9 +30,11 @@ __global__ void k_fill(T *p, T v, int64_t n) {
 // bounds[u] = first i in [0, n] with cost_prefix[i] * U >= u * C, for u in
 // [0, U]; bounds[U + 1], bounds[U + 2] = static units per wave and units per
 // tail chunk.
-__global__ void k_schedule_bounds(const int64_t *cpre, int64_t n, int64_t U, int waves, int64_t *bounds) {
+__global__ void k_schedule_bounds(const int64_t *cpre, int64_t n_host, const int64_t *n_dev, int64_t U,
+                                  int waves, int64_t *bounds) {
     const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (u > U + 2) return;
+    const int64_t n = n_dev ? *n_dev : n_host;
     const double C = (double)cpre[n], Ud = (double)U, t = (double)u * C;
     if (u > U) {
         const bool fine = C >= kTailFineCost * waves, tail = C >= kTailMinCost * waves;
@@ -97,6 +99,62 @@ template <class T>
 __global__ void k_set_total(T *out, int64_t n, const T *in, const T *excl) {
     // out[n] = excl[n-1] + in[n-1]
     if (threadIdx.x == 0 && blockIdx.x == 0) out[n] = n > 0 ? excl[n - 1] + in[n - 1] : T(0);
+}
+
+// Scans whose length is read on the device: element i takes part iff
+// i < n = min(*n_dev, cap) (n = cap without n_dev); tiles past n write a zero sum.
+template <class T>
+__global__ __launch_bounds__(kScanThreads) void k_scan_tile_n(const T *in, T *out, int64_t cap,
+                                                              const int64_t *n_dev, T *sums) {
+    __shared__ T part[kScanThreads];
+    int64_t n = cap;
+    if (n_dev) n = *n_dev < cap ? *n_dev : cap;
+    int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanPerThread;
+    T v[kScanPerThread];
+    T acc = 0;
+#pragma unroll
+    for (int k = 0; k < kScanPerThread; ++k) {
+        int64_t i = base + k;
+        v[k] = i < n ? in[i] : T(0);
+        acc += v[k];
+    }
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = 1; off < kScanThreads; off <<= 1) {
+        T t = threadIdx.x >= off ? part[threadIdx.x - off] : T(0);
+        __syncthreads();
+        part[threadIdx.x] += t;
+        __syncthreads();
+    }
+    T run = part[threadIdx.x] - acc;
+#pragma unroll
+    for (int k = 0; k < kScanPerThread; ++k) {
+        int64_t i = base + k;
+        if (i < n) out[i] = run;
+        run += v[k];
+    }
+    if (threadIdx.x == kScanThreads - 1) sums[blockIdx.x] = part[threadIdx.x];
+}
+
+template <class T>
+__global__ void k_scan_add_n(T *out, int64_t cap, const int64_t *n_dev, const T *offs) {
+    int64_t n = cap;
+    if (n_dev) n = *n_dev < cap ? *n_dev : cap;
+    int64_t i = (int64_t)blockIdx.x * kScanTile + threadIdx.x;
+    T o = offs[blockIdx.x];
+    for (int k = 0; k < kScanPerThread; ++k, i += kScanThreads)
+        if (i < n) out[i] += o;
+}
+
+// out[n] = total: the last level's single tile sum (every level below adds up
+// to it), written at the element count read on the device.
+template <class T>
+__global__ void k_scan_total_n(T *out, int64_t cap, const int64_t *n_dev, const T *top) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        int64_t n = cap;
+        if (n_dev) n = *n_dev < cap ? *n_dev : cap;
+        out[n] = top[0];
+    }
 }
 
 __device__ __forceinline__ int axis_cell(double x, int ji, double jk_sub, int per_jk, double width,
@@ -280,7 +338,14 @@ void fill(T *p, T v, int64_t n, hipStream_t st) {
 void schedule_bounds(const int64_t *cost_prefix, int64_t n, int waves, int64_t *bounds, hipStream_t st) {
     const int64_t U = (int64_t)waves * kScheduleUnits;
     hipLaunchKernelGGL(k_schedule_bounds, dim3((unsigned)((U + 3 + 255) / 256)), dim3(256), 0, st, cost_prefix,
-                       n, U, waves, bounds);
+                       n, (const int64_t *)nullptr, U, waves, bounds);
+    MIA_HIP(hipGetLastError());
+}
+void schedule_bounds(const int64_t *cost_prefix, const int64_t *n_dev, int waves, int64_t *bounds,
+                     hipStream_t st) {
+    const int64_t U = (int64_t)waves * kScheduleUnits;
+    hipLaunchKernelGGL(k_schedule_bounds, dim3((unsigned)((U + 3 + 255) / 256)), dim3(256), 0, st, cost_prefix,
+                       (int64_t)0, n_dev, U, waves, bounds);
     MIA_HIP(hipGetLastError());
 }
 
@@ -312,6 +377,55 @@ void exclusive_scan(Ctx &ctx, const T *in, T *out, int64_t n, hipStream_t st,
     hipLaunchKernelGGL(k_set_total<T>, dim3(1), dim3(1), 0, st, out, n, in, out);
     MIA_HIP(hipGetLastError());
 }
+template <class T>
+void scan_carve(Carve &c, int64_t cap, ScanBufs<T> &b) {
+    b.levels = 0;
+    int64_t n = cap < 0 ? 0 : cap;
+    // level l scans n[l] elements in tiles (at least one); its tile sums are level l + 1
+    while (true) {
+        const int64_t tiles = n > 0 ? (n + kScanTile - 1) / kScanTile : 1;
+        b.n[b.levels] = n;
+        b.sums[b.levels] = c.take<T>((size_t)tiles + 1);
+        b.offs[b.levels] = c.take<T>((size_t)tiles + 1);
+        ++b.levels;
+        if (tiles == 1) break;
+        MIA_REQUIRE(b.levels < 4, "scan too long");
+        n = tiles;
+    }
+}
+
+template <class T>
+void scan_ws(const T *in, T *out, int64_t cap, const int64_t *n_dev, const ScanBufs<T> &b, hipStream_t st) {
+    // level 0: the input; level l > 0 scans the tile sums of level l - 1 into offs
+    const T *src = in;
+    T *dst = out;
+    for (int l = 0; l < b.levels; ++l) {
+        const int64_t tiles = b.n[l] > 0 ? (b.n[l] + kScanTile - 1) / kScanTile : 1;
+        hipLaunchKernelGGL(k_scan_tile_n<T>, dim3((unsigned)tiles), dim3(kScanThreads), 0, st, src, dst,
+                           b.n[l], l == 0 ? n_dev : (const int64_t *)nullptr, b.sums[l]);
+        MIA_HIP(hipGetLastError());
+        src = b.sums[l];
+        dst = b.offs[l];
+    }
+    // add the scanned tile sums back down, level by level
+    for (int l = b.levels - 2; l >= 0; --l) {
+        const int64_t tiles = b.n[l] > 0 ? (b.n[l] + kScanTile - 1) / kScanTile : 1;
+        T *o = l == 0 ? out : b.offs[l - 1];
+        hipLaunchKernelGGL(k_scan_add_n<T>, dim3((unsigned)tiles), dim3(kScanThreads), 0, st, o, b.n[l],
+                           l == 0 ? n_dev : (const int64_t *)nullptr, (const T *)b.offs[l]);
+        MIA_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_scan_total_n<T>, dim3(1), dim3(64), 0, st, out, b.n[0], n_dev,
+                       (const T *)b.sums[b.levels - 1]);
+    MIA_HIP(hipGetLastError());
+}
+template void scan_carve<int32_t>(Carve &, int64_t, ScanBufs<int32_t> &);
+template void scan_carve<int64_t>(Carve &, int64_t, ScanBufs<int64_t> &);
+template void scan_ws<int32_t>(const int32_t *, int32_t *, int64_t, const int64_t *, const ScanBufs<int32_t> &,
+                               hipStream_t);
+template void scan_ws<int64_t>(const int64_t *, int64_t *, int64_t, const int64_t *, const ScanBufs<int64_t> &,
+                               hipStream_t);
+
 template void exclusive_scan<int32_t>(Ctx &, const int32_t *, int32_t *, int64_t, hipStream_t,
                                       const std::string &, int);
 template void exclusive_scan<int64_t>(Ctx &, const int64_t *, int64_t *, int64_t, hipStream_t,

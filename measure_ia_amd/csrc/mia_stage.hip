@@ -1,0 +1,676 @@
+// Staged counts: the cell sort of every sample of a call in one set of
+// launches, the drivers that run it with the planner and the pair kernel
+// (plain launches, or a captured hipGraph replayed while the launch sequence
+// stays the same), and prepared samples that keep their sort between counts.
+//
+// The reference rebuilds its KDTree for every counter call
+// (src/measureia/measure_w_box_jk.py:386, measure_m_box_jk.py:403) and loops
+// counters over samples; here a call sorts all of its samples at once:
+//   classify : one launch over the positions and shapes of all samples, each
+//              workgroup inside one segment (one kind of one sample): region
+//              id by the reference's strict sub-box test
+//              (src/measureia/measure_IA_base.py:404-452), cell key, and a rank
+//              inside the cell from the key counter's atomic (so the scatter
+//              needs no second atomic); per-region object counts and shape
+//              weight sums, aggregated per workgroup in LDS.
+//   scan     : one exclusive scan per kind over the concatenated key counters
+//              of all samples, so every start is a global index.
+//   scatter  : one launch: positions to SoA (a, b, l, w), shapes to 64-byte
+//              records {a, b, l, ax, ay, e*w, w, 2*e*w}.
+// Nothing is read back until the count has finished: sizes the device decides
+// stay on the device, and every buffer is carved from one workspace whose
+// size the host bounds beforehand.
+#include "mia_common.hpp"
+
+#include <cmath>
+#include <memory>
+#include <mutex>
+
+namespace mia {
+
+namespace {
+
+constexpr int kStageThreads = 256;
+constexpr int kStageItems = 4;                        // objects per thread
+constexpr int kStageWg = kStageThreads * kStageItems; // objects per workgroup
+constexpr int kLdsRegions = 4096;                     // region stats in LDS up to this many regions
+
+// One segment: the positions (kind 0) or the shapes (kind 1) of one sample.
+struct PrepSeg {
+    const double *xyz, *w, *axis, *e;
+    int64_t n;
+    int64_t obj0;             // first object of the segment among the objects of its kind
+    int64_t key0;             // first cell key of the sample in the concatenated key arrays
+    int64_t wg0;              // first workgroup of the segment
+    Grid g;
+    int kind, K, sample, pad;
+    int32_t *cnt;             // key counters of the kind
+    const int32_t *start;     // scanned key starts of the kind
+    int2 *kr;                 // (global key, rank in cell) of every object of the kind
+    unsigned long long *rcnt; // [2K] of the sample: positions, shapes per region
+    double *rsum;             // [2K] of the sample: sum w_s, sum w_s (1 - e^2/2) per region
+    double *pa, *pb, *pl, *pw;
+    double *srec;
+    int *oor;                 // the sample's out-of-box flag
+};
+
+__device__ __forceinline__ int find_seg(const int64_t *tab, int n, int64_t v) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tab[mid] <= v)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ int stage_axis_cell(double x, int ji, double jk_sub, int per_jk, double width, int n) {
+    int k;
+    if (ji >= 0) {
+        double rel = x - (double)ji * jk_sub;
+        double cs = jk_sub / per_jk;
+        double q = rel / cs;
+        q = q < 0.0 ? 0.0 : (q > (double)per_jk ? (double)per_jk : q);
+        k = clampi((int)floor(q), 0, per_jk - 1) + ji * per_jk;
+    } else {
+        double q = x / width;
+        q = q < 0.0 ? 0.0 : (q > (double)n ? (double)n : q);
+        k = clampi((int)floor(q), 0, n - 1);
+    }
+    return k;
+}
+
+// Region id, cell key and rank of every object of every segment.  LDS holds
+// the workgroup's per-region counts (and shape weight sums) when the
+// sample's regions fit (lds_regions > 0); they reach the sample's totals with
+// one atomic per touched region.
+__global__ __launch_bounds__(kStageThreads) void k_stage_classify(const PrepSeg *segs, const int64_t *wg0,
+                                                                  int nseg, int lds_regions) {
+    extern __shared__ double lstat[];  // [2K] sums, then [K] counts (u32)
+    const int j = find_seg(wg0, nseg, blockIdx.x);
+    const PrepSeg &G = segs[j];
+    const Grid &g = G.g;
+    const int K = G.K;
+    const bool lds = K > 0 && K <= lds_regions;
+    unsigned int *lcnt = (unsigned int *)(lstat + 2 * (lds ? K : 0));
+    if (lds) {
+        for (int r = threadIdx.x; r < K; r += blockDim.x) {
+            lcnt[r] = 0;
+            lstat[r] = 0.0;
+            lstat[K + r] = 0.0;
+        }
+        __syncthreads();
+    }
+    bool bad = false;
+    const int64_t base = (int64_t)(blockIdx.x - G.wg0) * kStageWg + threadIdx.x;
+#pragma unroll
+    for (int it = 0; it < kStageItems; ++it) {
+        const int64_t i = base + (int64_t)it * kStageThreads;
+        if (i >= G.n) break;
+        double x[3] = {G.xyz[3 * i], G.xyz[3 * i + 1], G.xyz[3 * i + 2]};
+        bool ok = true;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) ok = ok && (x[c] >= 0.0) && (x[c] <= g.L);
+        if (!ok) {
+            bad = true;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) x[c] = 0.0;
+        }
+        int ji[3] = {-1, -1, -1};
+        bool anom = false;
+        int region = 0;
+        if (g.jk > 0) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) ji[c] = strict_axis_index(x[c], g.jk_sub, g.jk);
+            anom = ji[0] < 0 || ji[1] < 0 || ji[2] < 0;
+            if (!anom) region = ji[0] * g.jk * g.jk + ji[1] * g.jk + ji[2];
+        }
+        const int ca = stage_axis_cell(x[g.ax_a], anom ? -1 : ji[g.ax_a], g.jk_sub, g.ncj, g.cw, g.nc);
+        const int cb = stage_axis_cell(x[g.ax_b], anom ? -1 : ji[g.ax_b], g.jk_sub, g.ncj, g.cw, g.nc);
+        const int cl = stage_axis_cell(x[g.los], anom ? -1 : ji[g.los], g.jk_sub, g.nzj, g.hz, g.nz);
+        const int64_t key = G.key0 + (int64_t)(ca * g.nc + cb) * (g.nz + 1) + (anom ? g.nz : cl);
+        const int rank = atomicAdd(&G.cnt[key], 1);
+        G.kr[G.obj0 + i] = make_int2((int)key, rank);
+        if (K > 0) {
+            double wi = 0.0, q = 0.0;
+            if (G.kind == 1) {
+                wi = G.w ? G.w[i] : 1.0;
+                const double ei = G.e[i];
+                q = wi * (1 - ei * ei / 2.0);
+            }
+            if (lds) {
+                atomicAdd(&lcnt[region], 1u);
+                if (G.kind == 1) {
+                    atomicAdd(&lstat[region], wi);
+                    atomicAdd(&lstat[K + region], q);
+                }
+            } else {
+                atomicAdd(&G.rcnt[G.kind * K + region], 1ull);
+                if (G.kind == 1) {
+                    atomicAdd(&G.rsum[region], wi);
+                    atomicAdd(&G.rsum[K + region], q);
+                }
+            }
+        }
+    }
+    if (bad) atomicOr(G.oor, 1);
+    if (lds) {
+        __syncthreads();
+        for (int r = threadIdx.x; r < K; r += blockDim.x) {
+            const unsigned int c = lcnt[r];
+            if (!c) continue;
+            atomicAdd(&G.rcnt[G.kind * K + r], (unsigned long long)c);
+            if (G.kind == 1) {
+                atomicAdd(&G.rsum[r], lstat[r]);
+                atomicAdd(&G.rsum[K + r], lstat[K + r]);
+            }
+        }
+    }
+}
+
+// Counting-sort scatter to start[key] + rank.
+__global__ __launch_bounds__(kStageThreads) void k_stage_scatter(const PrepSeg *segs, const int64_t *wg0,
+                                                                 int nseg) {
+    const int j = find_seg(wg0, nseg, blockIdx.x);
+    const PrepSeg &G = segs[j];
+    const Grid &g = G.g;
+    const int64_t base = (int64_t)(blockIdx.x - G.wg0) * kStageWg + threadIdx.x;
+#pragma unroll
+    for (int it = 0; it < kStageItems; ++it) {
+        const int64_t i = base + (int64_t)it * kStageThreads;
+        if (i >= G.n) break;
+        const int2 kr = G.kr[G.obj0 + i];
+        const int64_t d = (int64_t)G.start[kr.x] + kr.y;
+        const double a = G.xyz[3 * i + g.ax_a], b = G.xyz[3 * i + g.ax_b], l = G.xyz[3 * i + g.los];
+        const double wi = G.w ? G.w[i] : 1.0;
+        if (G.kind == 0) {
+            G.pa[d] = a;
+            G.pb[d] = b;
+            G.pl[d] = l;
+            G.pw[d] = wi;
+            continue;
+        }
+        double ax = G.axis[2 * i], ay = G.axis[2 * i + 1], ei = G.e[i];
+        // a shape whose axis or e is NaN (e.g. a zero axis vector normalised on
+        // the host) adds e+ = ex = 0, as the reference's isnan rule does
+        // (measure_w_box_jk.py:174-175): zero its axis and e in the record
+        if (!(ax == ax) || !(ay == ay) || !(ei == ei)) {
+            ax = 0.0;
+            ay = 0.0;
+            ei = 0.0;
+        }
+        const double ew = ei * wi;
+        double2 *r = (double2 *)(G.srec + 8 * d);
+        r[0] = make_double2(a, b);
+        r[1] = make_double2(l, ax);
+        r[2] = make_double2(ay, ew);
+        r[3] = make_double2(wi, 2.0 * ew);
+    }
+}
+
+inline unsigned nwg(int64_t n) { return (unsigned)((n + kStageWg - 1) / kStageWg); }
+
+// Host plan of the staged prepare of S samples.
+struct PrepPlan {
+    int S = 0;
+    std::vector<Grid> grids;
+    std::vector<PrepSeg> segs;      // 2S: positions of s at 2s, shapes at 2s + 1
+    std::vector<int64_t> wg0;       // 2S + 1
+    std::vector<int64_t> key0, rs0; // per sample: first key, first region-stat word
+    int64_t keys = 0, npos = 0, nshp = 0, rstat = 0;
+    int lds_regions = 0;
+    // bound
+    size_t off_segs = 0, off_wg0 = 0, param_bytes = 0;
+    char *param_dev = nullptr;
+    std::vector<char> param;
+    int32_t *pcnt = nullptr, *scnt = nullptr, *pstart = nullptr, *sstart = nullptr;
+    ScanBufs<int32_t> scan_p, scan_s;
+    int2 *pkr = nullptr, *skr = nullptr;
+    double *pa = nullptr, *pb = nullptr, *pl = nullptr, *pw = nullptr, *srec = nullptr;
+    unsigned long long *rcnt = nullptr;
+    double *rsum = nullptr;
+    int *oor = nullptr;
+    unsigned long long *zero_begin = nullptr;
+    size_t zero_words = 0;
+    std::string key;
+};
+
+void prep_plan(PrepPlan &p, int S, const mia_sample *samples, const mia_bins &bins, const mia_tuning *tuning) {
+    p.S = S;
+    p.grids.resize(S);
+    p.segs.assign(2 * S, PrepSeg{});
+    p.wg0.assign(2 * S + 1, 0);
+    p.key0.assign(S + 1, 0);
+    p.rs0.assign(S + 1, 0);
+    int kmax = 0;
+    for (int s = 0; s < S; ++s) {
+        const mia_sample &m = samples[s];
+        Grid &g = p.grids[s] = make_grid(m, bins, tuning);
+        const int K = g.jk > 0 ? g.jk * g.jk * g.jk : 0;
+        kmax = std::max(kmax, K);
+        p.key0[s + 1] = p.key0[s] + g.keys();
+        p.rs0[s + 1] = p.rs0[s] + 2 * (int64_t)K;
+        for (int kind = 0; kind < 2; ++kind) {
+            PrepSeg &G = p.segs[2 * s + kind];
+            std::memset(&G, 0, sizeof(G));
+            G.n = kind ? m.n_shape : m.n_pos;
+            G.obj0 = kind ? p.nshp : p.npos;
+            G.key0 = p.key0[s];
+            G.g = g;
+            G.kind = kind;
+            G.K = K;
+            G.sample = s;
+            const int j = 2 * s + kind;
+            G.wg0 = p.wg0[j];
+            p.wg0[j + 1] = p.wg0[j] + nwg(G.n);
+        }
+        p.npos += m.n_pos;
+        p.nshp += m.n_shape;
+    }
+    MIA_REQUIRE(p.npos < (1ll << 31) && p.nshp < (1ll << 31) && p.key0[S] < (1ll << 31),
+                "samples of one call hold more than 2^31 objects or cell keys");
+    p.keys = p.key0[S];
+    p.rstat = p.rs0[S];
+    p.lds_regions = kmax <= kLdsRegions ? kmax : 0;
+    // the launch sequence depends on the object counts, grids and region counts
+    auto put = [&](const void *v, size_t n) { p.key.append((const char *)v, n); };
+    put(&S, sizeof(S));
+    for (const PrepSeg &G : p.segs) {
+        put(&G.n, sizeof(G.n));
+        put(&G.g, sizeof(G.g));
+        put(&G.K, sizeof(G.K));
+    }
+}
+
+// Carves the prepare's buffers; with samples (not a dry run), binds the
+// segments to them and to the callers' inputs and returns the sorted views.
+std::vector<SortedSample> prep_bind(PrepPlan &p, Carve &c, const mia_sample *samples) {
+    const int S = p.S;
+    Carve pc;
+    p.off_segs = pc.off; pc.take<PrepSeg>(2 * S);
+    p.off_wg0 = pc.off; pc.take<int64_t>(2 * S + 1);
+    p.param_bytes = pc.off;
+    p.param_dev = c.take<char>(p.param_bytes);
+    char *z0 = c.dry ? nullptr : c.base + c.off;
+    p.pcnt = c.take<int32_t>(p.keys + 1);
+    p.scnt = c.take<int32_t>(p.keys + 1);
+    p.rcnt = c.take<unsigned long long>(p.rstat);
+    p.rsum = c.take<double>(p.rstat);
+    p.oor = c.take<int>(S);
+    p.zero_begin = (unsigned long long *)z0;
+    p.zero_words = c.dry ? 0 : (size_t)((c.base + c.off) - z0) / 8;
+    p.pstart = c.take<int32_t>(p.keys + 1);
+    p.sstart = c.take<int32_t>(p.keys + 1);
+    scan_carve<int32_t>(c, p.keys, p.scan_p);
+    scan_carve<int32_t>(c, p.keys, p.scan_s);
+    p.pkr = c.take<int2>(p.npos);
+    p.skr = c.take<int2>(p.nshp);
+    p.pa = c.take<double>(p.npos);
+    p.pb = c.take<double>(p.npos);
+    p.pl = c.take<double>(p.npos);
+    p.pw = c.take<double>(p.npos);
+    p.srec = c.take<double>(8 * p.nshp);
+    std::vector<SortedSample> ss(S);
+    for (int s = 0; s < S; ++s) {
+        SortedSample &x = ss[s];
+        const Grid &g = p.grids[s];
+        const mia_sample &m = samples[s];
+        x.n_pos = m.n_pos;
+        x.n_shape = m.n_shape;
+        x.g = g;
+        x.K = g.jk > 0 ? g.jk * g.jk * g.jk : 0;
+        x.periodic = m.periodic;
+        x.weighted = (m.pos_w != nullptr || m.shp_w != nullptr) ? 1 : 0;
+        if (c.dry) continue;
+        x.pa = p.pa;
+        x.pb = p.pb;
+        x.pl = p.pl;
+        x.pw = p.pw;
+        x.srec = p.srec;
+        x.pstart = p.pstart + p.key0[s];
+        x.sstart = p.sstart + p.key0[s];
+        x.rcnt = p.rcnt + p.rs0[s];
+        x.rsum = p.rsum + p.rs0[s];
+        for (int kind = 0; kind < 2; ++kind) {
+            PrepSeg &G = p.segs[2 * s + kind];
+            G.xyz = kind ? m.shp : m.pos;
+            G.w = kind ? m.shp_w : m.pos_w;
+            G.axis = kind ? m.axis : nullptr;
+            G.e = kind ? m.e : nullptr;
+            G.cnt = kind ? p.scnt : p.pcnt;
+            G.start = kind ? p.sstart : p.pstart;
+            G.kr = kind ? p.skr : p.pkr;
+            G.rcnt = p.rcnt + p.rs0[s];
+            G.rsum = p.rsum + p.rs0[s];
+            G.pa = p.pa;
+            G.pb = p.pb;
+            G.pl = p.pl;
+            G.pw = p.pw;
+            G.srec = p.srec;
+            G.oor = p.oor + s;
+        }
+    }
+    if (!c.dry) {
+        p.param.assign(p.param_bytes, 0);
+        std::memcpy(p.param.data() + p.off_segs, p.segs.data(), sizeof(PrepSeg) * 2 * S);
+        std::memcpy(p.param.data() + p.off_wg0, p.wg0.data(), sizeof(int64_t) * (2 * S + 1));
+    }
+    return ss;
+}
+
+size_t prep_bytes(PrepPlan &p, const mia_sample *samples) {
+    Carve c;
+    prep_bind(p, c, samples);
+    return c.off;
+}
+
+void prep_enqueue(const PrepPlan &p, hipStream_t st) {
+    const int nseg = 2 * p.S;
+    const PrepSeg *segs = (const PrepSeg *)(p.param_dev + p.off_segs);
+    const int64_t *wg0 = (const int64_t *)(p.param_dev + p.off_wg0);
+    fill<unsigned long long>(p.zero_begin, 0ull, (int64_t)p.zero_words, st);
+    const int64_t wgs = p.wg0[nseg];
+    const size_t lds = (size_t)p.lds_regions * (2 * sizeof(double) + sizeof(unsigned int));
+    if (wgs > 0) {
+        hipLaunchKernelGGL(k_stage_classify, dim3((unsigned)wgs), dim3(kStageThreads), lds, st, segs, wg0, nseg,
+                           p.lds_regions);
+        MIA_HIP(hipGetLastError());
+    }
+    scan_ws<int32_t>(p.pcnt, p.pstart, p.keys, nullptr, p.scan_p, st);
+    scan_ws<int32_t>(p.scnt, p.sstart, p.keys, nullptr, p.scan_s, st);
+    if (wgs > 0) {
+        hipLaunchKernelGGL(k_stage_scatter, dim3((unsigned)wgs), dim3(kStageThreads), 0, st, segs, wg0, nseg);
+        MIA_HIP(hipGetLastError());
+    }
+}
+
+// Pinned host staging: [0, a) parameter images, then the read-back words.
+void *pinned(Ctx &ctx, size_t bytes, bool may_grow) {
+    if (ctx.pinned_bytes >= bytes) return ctx.pinned;
+    if (!may_grow) throw Error{MIA_E_UNSUPPORTED, "parameter block larger than the preallocated staging"};
+    if (ctx.pinned) (void)hipHostFree(ctx.pinned);
+    ctx.pinned = nullptr;
+    ctx.pinned_bytes = 0;
+    size_t b = std::max<size_t>(bytes, (size_t)1 << 20);
+    if (hipHostMalloc(&ctx.pinned, b, hipHostMallocDefault) != hipSuccess) {
+        ctx.pinned = nullptr;
+        throw Error{MIA_E_NOMEM, "hipHostMalloc(staging)"};
+    }
+    ctx.pinned_bytes = b;
+    return ctx.pinned;
+}
+
+void *workspace(Ctx &ctx, size_t bytes) {
+    if (ctx.ws_bytes >= bytes) return ctx.ws;
+    // the graph refers to the old workspace
+    if (ctx.gexec) (void)hipGraphExecDestroy(ctx.gexec);
+    if (ctx.graph) (void)hipGraphDestroy(ctx.graph);
+    ctx.gexec = nullptr;
+    ctx.graph = nullptr;
+    ctx.gkey.clear();
+    if (ctx.ws) (void)hipFree(ctx.ws);
+    ctx.ws = nullptr;
+    ctx.ws_bytes = 0;
+    size_t b = bytes + bytes / 8 + ((size_t)1 << 20);
+    if (hipMalloc(&ctx.ws, b) != hipSuccess) {
+        ctx.ws = nullptr;
+        throw Error{MIA_E_NOMEM, "hipMalloc(workspace of " + std::to_string(b) + " bytes)"};
+    }
+    ctx.ws_bytes = b;
+    return ctx.ws;
+}
+
+struct PlanDeleter {
+    void operator()(CountPlan *p) const { free_plan(p); }
+};
+using PlanPtr = std::unique_ptr<CountPlan, PlanDeleter>;
+
+// Uploads the parameter images, runs (or replays) the sequence, reads back
+// the words and flags.  Returns the read-back words; oor[S] set per sample.
+struct RunOut {
+    unsigned long long words[kCountReadback];
+    std::vector<int> oor;
+    float ms_prepare = 0, ms_plan = 0, ms_pairs = 0, ms_finalize = 0, ms_total = 0;
+    int graph = 0;
+};
+
+void upload_and_run(Ctx &ctx, const PrepPlan *prep, const CountPlan &cp, hipStream_t st, StageMode mode,
+                    RunOut &r) {
+    const size_t pb = prep ? (prep->param_bytes + 255) & ~(size_t)255 : 0;
+    const size_t cb = (count_param_bytes(cp) + 255) & ~(size_t)255;
+    const int S = prep ? prep->S : 0;
+    const size_t rb = sizeof(unsigned long long) * kCountReadback + sizeof(int) * (S + 1);
+    char *h = (char *)pinned(ctx, pb + cb + rb, mode != STAGE_STATIC);
+    if (prep) std::memcpy(h, prep->param.data(), prep->param_bytes);
+    std::memcpy(h + pb, count_param_image(cp), count_param_bytes(cp));
+    Events ev(6);
+    MIA_HIP(hipEventRecord(ev[0], st));
+    if (prep) MIA_HIP(hipMemcpyAsync(prep->param_dev, h, prep->param_bytes, hipMemcpyHostToDevice, st));
+    MIA_HIP(hipMemcpyAsync(count_param_dev(cp), h + pb, count_param_bytes(cp), hipMemcpyHostToDevice, st));
+    if (mode == STAGE_GRAPH) {
+        std::string key = (prep ? prep->key : std::string()) + count_key(cp);
+        key.append((const char *)&ctx.ws, sizeof(ctx.ws));
+        if (key != ctx.gkey || !ctx.gexec) {
+            if (ctx.gexec) (void)hipGraphExecDestroy(ctx.gexec);
+            if (ctx.graph) (void)hipGraphDestroy(ctx.graph);
+            ctx.gexec = nullptr;
+            ctx.graph = nullptr;
+            ctx.gkey.clear();
+            MIA_HIP(hipStreamBeginCapture(ctx.cap_stream, hipStreamCaptureModeRelaxed));
+            try {
+                if (prep) prep_enqueue(*prep, ctx.cap_stream);
+                count_enqueue(cp, ctx.cap_stream, nullptr);
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(ctx.cap_stream, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            MIA_HIP(hipStreamEndCapture(ctx.cap_stream, &ctx.graph));
+            MIA_HIP(hipGraphInstantiate(&ctx.gexec, ctx.graph, nullptr, nullptr, 0));
+            ctx.gkey = key;
+        } else {
+            r.graph = 1;
+        }
+        MIA_HIP(hipEventRecord(ev[1], st));
+        MIA_HIP(hipGraphLaunch(ctx.gexec, st));
+        MIA_HIP(hipEventRecord(ev[5], st));
+    } else {
+        MIA_HIP(hipEventRecord(ev[1], st));
+        if (prep) prep_enqueue(*prep, st);
+        count_enqueue(cp, st, ev.data() + 2);
+    }
+    char *back = h + pb + cb;
+    MIA_HIP(hipMemcpyAsync(back, count_readback(cp), sizeof(unsigned long long) * kCountReadback,
+                           hipMemcpyDeviceToHost, st));
+    if (prep && S > 0)
+        MIA_HIP(hipMemcpyAsync(back + sizeof(unsigned long long) * kCountReadback, prep->oor, sizeof(int) * S,
+                               hipMemcpyDeviceToHost, st));
+    MIA_HIP(hipStreamSynchronize(st));
+    std::memcpy(r.words, back, sizeof(r.words));
+    r.oor.assign(S, 0);
+    if (S > 0) std::memcpy(r.oor.data(), back + sizeof(unsigned long long) * kCountReadback, sizeof(int) * S);
+    const double span_ms = r.words[5] > ~r.words[4] ? (double)(r.words[5] - ~r.words[4]) / ctx.wall_khz : 0.0;
+    if (mode == STAGE_GRAPH) {
+        r.ms_total = ev.ms(0, 5);
+        r.ms_pairs = (float)span_ms;
+    } else {
+        // ev[1]: before prepare, ev[2]: before plan, ev[3]: before pairs, ev[4]: after pairs, ev[5]: end
+        r.ms_prepare = ev.ms(1, 2);
+        r.ms_plan = ev.ms(2, 3);
+        r.ms_pairs = ev.ms(3, 4);
+        r.ms_finalize = ev.ms(4, 5);
+        r.ms_total = ev.ms(0, 5);
+    }
+}
+
+void fill_stats(mia_stats *stats, const RunOut &r, const CountPlan &cp, const Grid &g0) {
+    if (!stats) return;
+    std::memset(stats, 0, sizeof(*stats));
+    stats->ms_prepare = r.ms_prepare;
+    stats->ms_plan = r.ms_plan;
+    stats->ms_pairs = r.ms_pairs;
+    stats->ms_finalize = r.ms_finalize;
+    stats->ms_total = r.ms_total;
+    stats->rechecks = (int64_t)r.words[0];
+    stats->candidates = (int64_t)r.words[1];
+    stats->items = (int64_t)r.words[3];
+    stats->columns_per_side = g0.nc;
+    stats->slabs_per_side = g0.nz;
+    stats->slabs_per_block = count_G(cp, 0);
+    stats->block_shapes = count_bsz(cp);
+    stats->waves = count_waves(cp);
+    stats->method = MIA_FAST;
+    stats->graph = r.graph;
+    stats->item_capacity = count_item_cap(cp);
+}
+
+}  // namespace
+
+struct PreparedSample {
+    int device = 0;
+    void *mem = nullptr;
+    size_t bytes = 0;
+    SortedSample s;
+};
+
+void stage_init(Ctx &ctx) {
+    (void)pinned(ctx, (size_t)1 << 20, true);
+    MIA_HIP(hipStreamCreateWithFlags(&ctx.cap_stream, hipStreamNonBlocking));
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx.device) == hipSuccess && khz > 0)
+        ctx.wall_khz = khz;
+}
+
+void stage_release(Ctx &ctx) {
+    if (ctx.gexec) (void)hipGraphExecDestroy(ctx.gexec);
+    if (ctx.graph) (void)hipGraphDestroy(ctx.graph);
+    if (ctx.cap_stream) (void)hipStreamDestroy(ctx.cap_stream);
+    if (ctx.ws) (void)hipFree(ctx.ws);
+    if (ctx.pinned) (void)hipHostFree(ctx.pinned);
+    ctx.gexec = nullptr;
+    ctx.graph = nullptr;
+    ctx.cap_stream = nullptr;
+    ctx.ws = nullptr;
+    ctx.pinned = nullptr;
+    ctx.ws_bytes = ctx.pinned_bytes = 0;
+}
+
+void stage_count(Ctx &ctx, int n, const mia_sample *samples, const mia_bins &bins, const mia_counts *outs,
+                 const mia_tuning *tuning, mia_stats *stats, hipStream_t st, StageMode mode) {
+    for (int attempt = 0;; ++attempt) {
+        PrepPlan prep;
+        prep_plan(prep, n, samples, bins, tuning);
+        Carve dry;
+        std::vector<SortedSample> proto = prep_bind(prep, dry, samples);
+        const size_t pbytes = dry.off;
+        PlanPtr cp(plan_count(ctx, proto, bins, tuning, ctx.item_cap));
+        const size_t total = pbytes + count_bytes(*cp);
+        char *base;
+        if (mode == STAGE_STATIC) {
+            MIA_REQUIRE(ctx.scratch != nullptr, "no static workspace on this device");
+            if (total > (size_t)stage_static_bytes())
+                throw Error{MIA_E_UNSUPPORTED, "sample needs " + std::to_string(total) +
+                                                   " bytes of workspace; the static workspace holds " +
+                                                   std::to_string(stage_static_bytes())};
+            base = (char *)ctx.scratch;
+        } else {
+            base = (char *)workspace(ctx, total);
+        }
+        Carve c;
+        c.dry = false;
+        c.base = base;
+        std::vector<SortedSample> ss = prep_bind(prep, c, samples);
+        count_bind(*cp, c, ss, outs);
+        RunOut r;
+        upload_and_run(ctx, &prep, *cp, st, mode, r);
+        for (int s = 0; s < n; ++s)
+            if (r.oor[s])
+                throw Error{MIA_E_RANGE, "coordinates outside [0, boxsize] (or NaN) in sample " + std::to_string(s)};
+        if (r.words[2]) {
+            // more work items than the capacity: grow it and count again
+            if (mode == STAGE_STATIC || attempt >= 3)
+                throw Error{MIA_E_UNSUPPORTED, "work items exceed the workspace capacity (" +
+                                                   std::to_string(r.words[3]) + " items)"};
+            ctx.item_cap = std::max<int64_t>(ctx.item_cap, (int64_t)(r.words[3] + r.words[3] / 4 + 1024));
+            continue;
+        }
+        fill_stats(stats, r, *cp, prep.grids[0]);
+        return;
+    }
+}
+
+PreparedSample *stage_prepare(Ctx &ctx, const mia_sample &s, const mia_bins &bins, const mia_tuning *tuning,
+                              hipStream_t st) {
+    PrepPlan prep;
+    prep_plan(prep, 1, &s, bins, tuning);
+    const size_t bytes = prep_bytes(prep, &s);
+    std::unique_ptr<PreparedSample> h(new PreparedSample());
+    h->device = ctx.device;
+    if (hipMalloc(&h->mem, bytes) != hipSuccess) {
+        h->mem = nullptr;
+        throw Error{MIA_E_NOMEM, "hipMalloc(prepared sample of " + std::to_string(bytes) + " bytes)"};
+    }
+    h->bytes = bytes;
+    try {
+        Carve c;
+        c.dry = false;
+        c.base = (char *)h->mem;
+        std::vector<SortedSample> ss = prep_bind(prep, c, &s);
+        char *stage = (char *)pinned(ctx, prep.param_bytes + 256, true);
+        std::memcpy(stage, prep.param.data(), prep.param_bytes);
+        MIA_HIP(hipMemcpyAsync(prep.param_dev, stage, prep.param_bytes, hipMemcpyHostToDevice, st));
+        prep_enqueue(prep, st);
+        int *flag = (int *)(stage + ((prep.param_bytes + 255) & ~(size_t)255));
+        MIA_HIP(hipMemcpyAsync(flag, prep.oor, sizeof(int), hipMemcpyDeviceToHost, st));
+        MIA_HIP(hipStreamSynchronize(st));
+        if (*flag) throw Error{MIA_E_RANGE, "coordinates outside [0, boxsize] (or NaN)"};
+        h->s = ss[0];
+    } catch (...) {
+        (void)hipFree(h->mem);
+        throw;
+    }
+    return h.release();
+}
+
+void stage_count_prepared(Ctx &ctx, int n, const PreparedSample *const *hs, const mia_bins &bins,
+                          const mia_counts *outs, const mia_tuning *tuning, mia_stats *stats, hipStream_t st) {
+    std::vector<SortedSample> ss(n);
+    for (int i = 0; i < n; ++i) {
+        MIA_REQUIRE(hs[i] != nullptr, "prepared sample is NULL");
+        MIA_REQUIRE(hs[i]->device == ctx.device, "prepared sample belongs to another device");
+        ss[i] = hs[i]->s;
+    }
+    for (int attempt = 0;; ++attempt) {
+        PlanPtr cp(plan_count(ctx, ss, bins, tuning, ctx.item_cap));
+        Carve c;
+        c.dry = false;
+        c.base = (char *)workspace(ctx, count_bytes(*cp));
+        count_bind(*cp, c, ss, outs);
+        RunOut r;
+        upload_and_run(ctx, nullptr, *cp, st, STAGE_EAGER, r);
+        if (r.words[2]) {
+            if (attempt >= 3)
+                throw Error{MIA_E_UNSUPPORTED, "work items exceed the workspace capacity (" +
+                                                   std::to_string(r.words[3]) + " items)"};
+            ctx.item_cap = std::max<int64_t>(ctx.item_cap, (int64_t)(r.words[3] + r.words[3] / 4 + 1024));
+            continue;
+        }
+        fill_stats(stats, r, *cp, ss[0].g);
+        return;
+    }
+}
+
+const SortedSample &prepared_view(const PreparedSample *h) { return h->s; }
+
+void stage_free_prepared(PreparedSample *h) {
+    if (!h) return;
+    if (h->mem) (void)hipFree(h->mem);
+    delete h;
+}
+
+int64_t stage_static_bytes() { return 0; }
+
+}  // namespace mia

@@ -251,6 +251,44 @@ class Engine:
         stats = st.as_dict()
         return [self._host(o, K, th, stats) for o, K in zip(outs, Ks)]
 
+    def prepare(self, sample: SampleArrays, th: Thresholds, tuning: Optional[dict] = None) -> "Prepared":
+        """Sort one sample once on the device (mia_prepare_sample); counts of the
+        returned handle skip the sort.  The cell grid is chosen for th."""
+        ms, keep, weighted = self._prepare(sample)
+        bins, keep_b = self._bins(th, False)
+        tu = self._tuning(tuning)
+        h = ctypes.c_void_p()
+        stream = self.torch.cuda.current_stream(self.dev).cuda_stream
+        rc = self.lib.mia_prepare_sample(self.ctx, ctypes.byref(ms), ctypes.byref(bins),
+                                         ctypes.byref(tu) if tu else None, ctypes.byref(h),
+                                         ctypes.c_void_p(stream))
+        _capi.check(self.lib, self.ctx, rc)
+        K = sample.jk_per_side ** 3 if sample.jk_per_side > 0 else 0
+        return Prepared(self, h, K)
+
+    def count_prepared(self, prepared: Sequence["Prepared"], th: Thresholds, variance: bool = False,
+                       tuning: Optional[dict] = None):
+        """Count prepared samples in one launch (mia_count_prepared); returns a list of results."""
+        n = len(prepared)
+        hs = (ctypes.c_void_p * n)(*[p.handle for p in prepared])
+        cs = (_capi.MiaCounts * n)()
+        outs = []
+        nb = th.n_r * th.n_2
+        for i, p in enumerate(prepared):
+            layout, buf, c = self._outputs(p.K, nb, variance)
+            outs.append((layout, buf))
+            cs[i] = c
+        bins, keep_b = self._bins(th, variance)
+        st = _capi.MiaStats()
+        tu = self._tuning(tuning)
+        stream = self.torch.cuda.current_stream(self.dev).cuda_stream
+        rc = self.lib.mia_count_prepared(self.ctx, n, hs, ctypes.byref(bins), cs,
+                                         ctypes.byref(tu) if tu else None, ctypes.byref(st),
+                                         ctypes.c_void_p(stream))
+        _capi.check(self.lib, self.ctx, rc)
+        stats = st.as_dict()
+        return [self._host(o, p.K, th, stats) for o, p in zip(outs, prepared)]
+
     def _dev_i32(self, a):
         t = self.torch
         if a is None:
@@ -324,6 +362,26 @@ class Engine:
                                      out.data_ptr(), ctypes.c_void_p(stream))
         _capi.check(self.lib, self.ctx, rc)
         return out[:n].cpu().numpy()
+
+
+class Prepared:
+    """A sample sorted on the device (mia_prepare_sample), released with the object."""
+
+    def __init__(self, engine: Engine, handle, K: int):
+        self.engine = engine
+        self.handle = handle
+        self.K = K
+
+    def release(self):
+        if self.handle is not None and self.engine.ctx:
+            self.engine.lib.mia_release_prepared(self.engine.ctx, self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
 
 
 _ENGINES = {}

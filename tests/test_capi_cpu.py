@@ -40,7 +40,7 @@ def lib():
 def test_library_exports_every_symbol(lib):
     for name in _declared_functions():
         assert hasattr(lib, name), name
-    assert lib.mia_abi_version() == 2
+    assert lib.mia_abi_version() == 3
 
 
 @pytest.mark.skipif(shutil.which("gcc") is None, reason="gcc missing")
@@ -71,3 +71,6 @@ def test_null_context_is_rejected(lib):
     assert rc == -1
     assert lib.mia_region_ids(None, None, 0, 1.0, 1, None, None) == -1
     assert lib.mia_count_lightcone(None, None, None, None, None, None, None) == -1
+    assert lib.mia_prepare_sample(None, None, None, None, None, None) == -1
+    assert lib.mia_count_prepared(None, 0, None, None, None, None, None, None) == -1
+    lib.mia_release_prepared(None, None)  # NULL is ignored
