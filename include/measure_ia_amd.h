@@ -191,6 +191,22 @@ int mia_count_batch(mia_ctx *ctx, int32_t n, const mia_sample *samples, const mi
                     const mia_counts *outs, const mia_tuning *tuning, mia_stats *stats,
                     void *stream);
 
+/* mia_count (MIA_FAST) without any allocation.  This entry point has no
+ * counterpart in the reference; it is for callers that cannot allocate
+ * device memory (code under hipGraph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Every buffer of the count is carved from a static
+ * device workspace of mia_noalloc_workspace_bytes() bytes (256 MiB) that each
+ * device holds from mia_create until the process ends; the call makes no
+ * hipMalloc, hipFree, event, stream or graph.  A count that needs more
+ * workspace (about 1.5 M positions with 0.3 M shapes at the benchmark binning)
+ * returns MIA_E_UNSUPPORTED with the size it needs, before anything is
+ * launched where the size is known up front.  All calls on one device share
+ * its workspace: calls that overlap in time on one device (two host threads,
+ * or two streams) race and must be serialised by the caller.  No stats. */
+int mia_count_noalloc(mia_ctx *ctx, const mia_sample *sample, const mia_bins *bins,
+                      const mia_counts *out, const mia_tuning *tuning, void *stream);
+int64_t mia_noalloc_workspace_bytes(void);
+
 /* A sample whose cell sort is kept on the device between counts. */
 typedef struct mia_prepared mia_prepared;
 

@@ -98,6 +98,9 @@ SIGNATURES = {
     "mia_region_ids": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
                                       ctypes.c_double, ctypes.c_int32, ctypes.c_void_p,
                                       ctypes.c_void_p]),
+    "mia_count_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(MiaSample), ctypes.POINTER(MiaBins),
+                                         ctypes.POINTER(MiaCounts), ctypes.POINTER(MiaTuning), ctypes.c_void_p]),
+    "mia_noalloc_workspace_bytes": (ctypes.c_int64, []),
     "mia_prepare_sample": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(MiaSample), ctypes.POINTER(MiaBins),
                                           ctypes.POINTER(MiaTuning), ctypes.POINTER(ctypes.c_void_p),
                                           ctypes.c_void_p]),

@@ -210,6 +210,23 @@ int mia_count_batch(mia_ctx *ctx, int32_t n, const mia_sample *samples, const mi
     });
 }
 
+int mia_count_noalloc(mia_ctx *ctx, const mia_sample *sample, const mia_bins *bins, const mia_counts *out,
+                      const mia_tuning *tuning, void *stream) {
+    return guard(ctx, [&] {
+        check_sample(sample);
+        check_bins(bins);
+        bool weighted = sample->pos_w != nullptr || sample->shp_w != nullptr;
+        check_out(out, weighted);
+        if (!mia::fast_path_fits(*bins, weighted, sample->jk_per_side))
+            throw mia::Error{MIA_E_UNSUPPORTED, "binning or jackknife count too large for MIA_FAST"};
+        MIA_HIP(hipSetDevice(ctx->c.device));
+        mia::stage_count(ctx->c, 1, sample, *bins, out, tuning, nullptr, (hipStream_t)stream,
+                         mia::STAGE_STATIC);
+    });
+}
+
+int64_t mia_noalloc_workspace_bytes(void) { return mia::stage_static_bytes(); }
+
 int mia_prepare_sample(mia_ctx *ctx, const mia_sample *sample, const mia_bins *bins,
                        const mia_tuning *tuning, mia_prepared **out, void *stream) {
     return guard(ctx, [&] {

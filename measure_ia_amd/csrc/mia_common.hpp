@@ -286,8 +286,9 @@ struct CountCall;
 // Host plan of a staged count over n sorted samples: worst-case sizes,
 // launch shapes and the pair-kernel configuration.
 struct CountPlan;
+// item_cap: at least this many work-item slots (exactly this many with cap_exact)
 struct CountPlan *plan_count(Ctx &ctx, const std::vector<SortedSample> &ss, const mia_bins &bins,
-                             const mia_tuning *tuning, int64_t item_cap);
+                             const mia_tuning *tuning, int64_t item_cap, bool cap_exact);
 void free_plan(CountPlan *p);
 // workspace bytes of the count stages (after plan_count)
 size_t count_bytes(const CountPlan &p);

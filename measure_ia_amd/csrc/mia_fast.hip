@@ -1118,7 +1118,7 @@ void free_plan(CountPlan *p) { delete p; }
 double estimate_items(const Grid &g, int G, double reach_r, double reach_z, bool rmu, int64_t ns, int64_t np);
 
 CountPlan *plan_count(Ctx &ctx, const std::vector<SortedSample> &ss, const mia_bins &bins,
-                      const mia_tuning *tuning, int64_t item_cap) {
+                      const mia_tuning *tuning, int64_t item_cap, bool cap_exact) {
     pair_kernels_init();
     std::unique_ptr<CountPlan> P(new CountPlan());
     CountPlan &p = *P;
@@ -1206,7 +1206,7 @@ CountPlan *plan_count(Ctx &ctx, const std::vector<SortedSample> &ss, const mia_b
     p.plan_local = plan_lds <= 64 * 1024 ? 1 : 0;
     // item capacity: the caller's (grown after an overflow), at least 3x the
     // uniform-density estimate plus a margin
-    p.item_cap = std::max<int64_t>(item_cap, (int64_t)(3.0 * est_items) + 65536);
+    p.item_cap = cap_exact ? item_cap : std::max<int64_t>(item_cap, (int64_t)(3.0 * est_items) + 65536);
     p.item_cap = std::min<int64_t>(p.item_cap, (int64_t)1 << 31);
 
     // the launch-sequence key: everything the launches depend on except the

@@ -28,6 +28,14 @@
 
 namespace mia {
 
+// Workspace of the allocation-free entry point (mia_count_noalloc): every
+// buffer of one count is carved from this array, so that entry point never
+// allocates device memory.  Each device that loads the library holds its own
+// copy until the process ends.  In host code the name is only a stand-in;
+// the device address comes from hipGetSymbolAddress (stage_init).
+constexpr size_t kStaticBytes = (size_t)256 << 20;
+static __device__ __align__(256) unsigned char scratch[kStaticBytes];
+
 namespace {
 
 constexpr int kStageThreads = 256;
@@ -445,11 +453,22 @@ void upload_and_run(Ctx &ctx, const PrepPlan *prep, const CountPlan &cp, hipStre
     char *h = (char *)pinned(ctx, pb + cb + rb, mode != STAGE_STATIC);
     if (prep) std::memcpy(h, prep->param.data(), prep->param_bytes);
     std::memcpy(h + pb, count_param_image(cp), count_param_bytes(cp));
-    Events ev(6);
-    MIA_HIP(hipEventRecord(ev[0], st));
-    if (prep) MIA_HIP(hipMemcpyAsync(prep->param_dev, h, prep->param_bytes, hipMemcpyHostToDevice, st));
-    MIA_HIP(hipMemcpyAsync(count_param_dev(cp), h + pb, count_param_bytes(cp), hipMemcpyHostToDevice, st));
-    if (mode == STAGE_GRAPH) {
+    if (mode == STAGE_STATIC) {
+        // no events, streams or graphs: nothing is created or allocated
+        if (prep) MIA_HIP(hipMemcpyAsync(prep->param_dev, h, prep->param_bytes, hipMemcpyHostToDevice, st));
+        MIA_HIP(hipMemcpyAsync(count_param_dev(cp), h + pb, count_param_bytes(cp), hipMemcpyHostToDevice, st));
+        if (prep) prep_enqueue(*prep, st);
+        count_enqueue(cp, st, nullptr);
+    }
+    Events ev(mode == STAGE_STATIC ? 0 : 6);
+    if (mode != STAGE_STATIC) MIA_HIP(hipEventRecord(ev[0], st));
+    if (mode != STAGE_STATIC) {
+        if (prep) MIA_HIP(hipMemcpyAsync(prep->param_dev, h, prep->param_bytes, hipMemcpyHostToDevice, st));
+        MIA_HIP(hipMemcpyAsync(count_param_dev(cp), h + pb, count_param_bytes(cp), hipMemcpyHostToDevice, st));
+    }
+    if (mode == STAGE_STATIC) {
+        // enqueued above
+    } else if (mode == STAGE_GRAPH) {
         std::string key = (prep ? prep->key : std::string()) + count_key(cp);
         key.append((const char *)&ctx.ws, sizeof(ctx.ws));
         if (key != ctx.gkey || !ctx.gexec) {
@@ -493,7 +512,9 @@ void upload_and_run(Ctx &ctx, const PrepPlan *prep, const CountPlan &cp, hipStre
     r.oor.assign(S, 0);
     if (S > 0) std::memcpy(r.oor.data(), back + sizeof(unsigned long long) * kCountReadback, sizeof(int) * S);
     const double span_ms = r.words[5] > ~r.words[4] ? (double)(r.words[5] - ~r.words[4]) / ctx.wall_khz : 0.0;
-    if (mode == STAGE_GRAPH) {
+    if (mode == STAGE_STATIC) {
+        r.ms_pairs = (float)span_ms;
+    } else if (mode == STAGE_GRAPH) {
         r.ms_total = ev.ms(0, 5);
         r.ms_pairs = (float)span_ms;
     } else {
@@ -536,7 +557,27 @@ struct PreparedSample {
     SortedSample s;
 };
 
+namespace {
+std::mutex g_scratch_mu;
+void *g_scratch[64] = {};     // device address of `scratch` per device
+bool g_scratch_done[64] = {};
+}  // namespace
+
 void stage_init(Ctx &ctx) {
+    // the static workspace's device address, once per device, with that
+    // device current (mia_create has made it so)
+    MIA_REQUIRE(ctx.device >= 0 && ctx.device < 64, "device index above 63");
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        if (!g_scratch_done[ctx.device]) {
+            void *ptr = nullptr;
+            MIA_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+            MIA_REQUIRE(ptr != nullptr, "no device address for the static workspace");
+            g_scratch[ctx.device] = ptr;
+            g_scratch_done[ctx.device] = true;
+        }
+        ctx.scratch = g_scratch[ctx.device];
+    }
     (void)pinned(ctx, (size_t)1 << 20, true);
     MIA_HIP(hipStreamCreateWithFlags(&ctx.cap_stream, hipStreamNonBlocking));
     int khz = 0;
@@ -566,15 +607,26 @@ void stage_count(Ctx &ctx, int n, const mia_sample *samples, const mia_bins &bin
         Carve dry;
         std::vector<SortedSample> proto = prep_bind(prep, dry, samples);
         const size_t pbytes = dry.off;
-        PlanPtr cp(plan_count(ctx, proto, bins, tuning, ctx.item_cap));
-        const size_t total = pbytes + count_bytes(*cp);
+        PlanPtr cp(plan_count(ctx, proto, bins, tuning, ctx.item_cap, false));
+        size_t total = pbytes + count_bytes(*cp);
         char *base;
         if (mode == STAGE_STATIC) {
             MIA_REQUIRE(ctx.scratch != nullptr, "no static workspace on this device");
-            if (total > (size_t)stage_static_bytes())
-                throw Error{MIA_E_UNSUPPORTED, "sample needs " + std::to_string(total) +
+            const size_t room = (size_t)stage_static_bytes();
+            if (total > room) {
+                // fewer work-item slots (32 bytes each with their costs and
+                // scan), down to a floor; a count that needs more reports it
+                const int64_t over = (int64_t)((total - room) / 32) + 4096;
+                const int64_t cap = count_item_cap(*cp) - over;
+                if (cap >= 65536) {
+                    cp.reset(plan_count(ctx, proto, bins, tuning, cap, true));
+                    total = pbytes + count_bytes(*cp);
+                }
+            }
+            if (total > room)
+                throw Error{MIA_E_UNSUPPORTED, "the count needs " + std::to_string(total) +
                                                    " bytes of workspace; the static workspace holds " +
-                                                   std::to_string(stage_static_bytes())};
+                                                   std::to_string(room) + " (mia_count allocates what it needs)"};
             base = (char *)ctx.scratch;
         } else {
             base = (char *)workspace(ctx, total);
@@ -591,7 +643,12 @@ void stage_count(Ctx &ctx, int n, const mia_sample *samples, const mia_bins &bin
                 throw Error{MIA_E_RANGE, "coordinates outside [0, boxsize] (or NaN) in sample " + std::to_string(s)};
         if (r.words[2]) {
             // more work items than the capacity: grow it and count again
-            if (mode == STAGE_STATIC || attempt >= 3)
+            if (mode == STAGE_STATIC)
+                throw Error{MIA_E_UNSUPPORTED, "the count needs " + std::to_string(r.words[3]) +
+                                                   " work items; the static workspace holds " +
+                                                   std::to_string(count_item_cap(*cp)) +
+                                                   " (mia_count allocates what it needs)"};
+            if (attempt >= 3)
                 throw Error{MIA_E_UNSUPPORTED, "work items exceed the workspace capacity (" +
                                                    std::to_string(r.words[3]) + " items)"};
             ctx.item_cap = std::max<int64_t>(ctx.item_cap, (int64_t)(r.words[3] + r.words[3] / 4 + 1024));
@@ -644,7 +701,7 @@ void stage_count_prepared(Ctx &ctx, int n, const PreparedSample *const *hs, cons
         ss[i] = hs[i]->s;
     }
     for (int attempt = 0;; ++attempt) {
-        PlanPtr cp(plan_count(ctx, ss, bins, tuning, ctx.item_cap));
+        PlanPtr cp(plan_count(ctx, ss, bins, tuning, ctx.item_cap, false));
         Carve c;
         c.dry = false;
         c.base = (char *)workspace(ctx, count_bytes(*cp));
@@ -671,6 +728,6 @@ void stage_free_prepared(PreparedSample *h) {
     delete h;
 }
 
-int64_t stage_static_bytes() { return 0; }
+int64_t stage_static_bytes() { return (int64_t)kStaticBytes; }
 
 }  // namespace mia

@@ -21,7 +21,7 @@ HEADER = os.path.join(ROOT, "include", "measure_ia_amd.h")
 
 def _declared_functions():
     text = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:int|void|const char \*)\s*(mia_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|int64_t|void|const char \*)\s*(mia_\w+)\s*\(", text, re.M)))
 
 
 def test_header_functions_are_bound():
@@ -74,3 +74,5 @@ def test_null_context_is_rejected(lib):
     assert lib.mia_prepare_sample(None, None, None, None, None, None) == -1
     assert lib.mia_count_prepared(None, 0, None, None, None, None, None, None) == -1
     lib.mia_release_prepared(None, None)  # NULL is ignored
+    assert lib.mia_count_noalloc(None, None, None, None, None, None) == -1
+    assert lib.mia_noalloc_workspace_bytes() == 256 << 20
