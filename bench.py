@@ -355,10 +355,35 @@ def main():
     ms_step = elapsed / a.steps * 1e3
     pairs = int(sum(r.npairs[0].sum() for r in res))
     value = pairs / (ms_step / 1e3)
-    k_ms = float(np.mean([s["ms_pairs"] for s in kstats]))
+    # pair-kernel and sort times from HIP events on the launch stream; a
+    # graph-replayed batch has no events between its launches, so these come
+    # from the same count with plain launches
+    ev_stats = kstats
+    graph_pairs_ms = None
+    if batch:
+        graph_pairs_ms = float(np.mean([s["ms_pairs"] for s in kstats]))  # wall-clock span inside the graph
+        ev_stats = []
+        for _ in range(max(3, min(a.steps, 5))):
+            ev_stats.append(eng.count_batch(samples, th, tuning={"graph": -1})[0].stats)
+    k_ms = float(np.mean([s["ms_pairs"] for s in ev_stats]))
     flops = pairs * FLOP_PER_PAIR[w["mode"]]
     achieved = flops / (k_ms / 1e3) / 1e12
     log(f"fast: {ms_step:.2f} ms/step, pair kernel {k_ms:.2f} ms, {pairs} pairs")
+    prepared = None
+    if not batch:
+        # the same count on a prepared sample: the sort is done once beforehand
+        h = eng.prepare(samples[0], th)
+        eng.count_prepared([h], th)
+        torch.cuda.synchronize()
+        tp = time.perf_counter()
+        for _ in range(a.steps):
+            rp_ = eng.count_prepared([h], th)[0]
+        torch.cuda.synchronize()
+        assert np.array_equal(rp_.npairs, res[0].npairs), "prepared and unprepared counts differ"
+        prepared = dict(ms_per_step=(time.perf_counter() - tp) / a.steps * 1e3, ms_plan=rp_.stats["ms_plan"],
+                        ms_pairs=rp_.stats["ms_pairs"])
+        h.release()
+        log(f"prepared sample: {prepared['ms_per_step']:.2f} ms/step")
 
     loop = None
     if batch:
@@ -429,7 +454,7 @@ def main():
 
     st = res[0].stats
     # the counting sort (region ids, cell keys, scan, scatter of every sample) against HBM
-    sort_ms = float(np.mean([s["ms_prepare"] for s in kstats]))
+    sort_ms = float(np.mean([s["ms_prepare"] for s in ev_stats]))
     sort_bytes = sum(SORT_BYTES[0] * p.n_pos + SORT_BYTES[1] * p.n_shape for p in preps)
     sort_gbps = sort_bytes / (sort_ms / 1e3) / 1e9
     cell_sort = {"bound": "hbm", "achieved": sort_gbps, "peak": PEAK_HBM_GBPS, "unit": "GB/s",
@@ -454,6 +479,10 @@ def main():
         "cpu_baseline": cpu,
         "plain_gpu": plain,
         "one_call_per_sample": loop,
+        "graph_replayed": bool(st.get("graph")),
+        "graph_pair_kernel_ms_wallclock": graph_pairs_ms,
+        "step_over_pair_kernel": ms_step / k_ms,
+        "prepared_sample": prepared,
         "speedup_vs_plain": (value / plain["pairs_per_s"]) if plain else None,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
         "s_per_wgg_wgplus": e2e,

@@ -135,7 +135,7 @@ typedef struct mia_tuning {
     int32_t block_shapes;       /* shapes per block, <= 64                            */
     int32_t item_positions;     /* max positions per work item                        */
     int32_t waves;              /* persistent waves of the pair kernel (at most 2^20)  */
-    int32_t hist_copies;        /* interleaved LDS copies of each histogram bin (1,2,4) */
+    int32_t hist_copies;        /* ignored (one LDS histogram copy per wave)           */
     int32_t cell_shapes;        /* > 0: size cells to hold this many shapes on average
                                    instead of the cost-model search                     */
     int32_t r_bins_per_pass;    /* > 0: r bins per pass of the pair kernel, clamped to
@@ -164,6 +164,8 @@ typedef struct mia_stats {
     int32_t graph;          /* 1: the call replayed a captured hipGraph                 */
     int32_t reserved;
     int64_t item_capacity;  /* work-item slots of the workspace                         */
+    int64_t wave_iterations;    /* pair-kernel loop trips: one 64-lane position batch
+                                   against one shape                                    */
 } mia_stats;
 
 int mia_abi_version(void);

@@ -71,7 +71,7 @@ class MiaStats(ctypes.Structure):
                 ("slabs_per_block", ctypes.c_int32), ("block_shapes", ctypes.c_int32),
                 ("waves", ctypes.c_int32), ("method", ctypes.c_int32),
                 ("ms_total", ctypes.c_double), ("graph", ctypes.c_int32), ("reserved", ctypes.c_int32),
-                ("item_capacity", ctypes.c_int64)]
+                ("item_capacity", ctypes.c_int64), ("wave_iterations", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

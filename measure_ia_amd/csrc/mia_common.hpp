@@ -303,10 +303,12 @@ void *count_param_dev(const CountPlan &p);
 // (4 events), records them before the plan, before the pairs, after the
 // pairs and at the end.
 void count_enqueue(const CountPlan &p, hipStream_t st, const hipEvent_t *ev);
-// Device words read back after a count: [0] rechecks, [1] candidates,
-// [2] item overflow, [3] items, [4] pair-kernel start (~wall clock), [5] end.
+// Device words read back after a count: [0] rechecks, [1] candidates of the
+// planned items, [2] item overflow, [3] items, [4] pair-kernel start (~wall
+// clock), [5] end, [6] wave-iterations of the pair kernel (one 64-lane
+// position batch against one shape).
 const unsigned long long *count_readback(const CountPlan &p);
-constexpr int kCountReadback = 6;
+constexpr int kCountReadback = 7;
 int count_waves(const CountPlan &p);
 int count_bsz(const CountPlan &p);
 int count_G(const CountPlan &p, int sample);

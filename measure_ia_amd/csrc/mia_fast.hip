@@ -41,7 +41,7 @@ constexpr int kMaxWavesPerBlock = 4;
 constexpr int kThreads = kWave * kMaxWavesPerBlock;
 constexpr size_t kLdsBytes = 160 * 1024;         // LDS per CU (gfx950)
 constexpr int64_t kMaxKeysPerSample = 1 << 24;   // K^2 planner keys per sample (K regions)
-constexpr double kSin2Exact = 1e-10;  // FP64 arccos-domain check below this sin^2
+constexpr double kSinExact = 1e-5;  // FP64 arccos-domain check when |sin phi| < 1e-5 |cos phi|
 
 struct Block {
     int32_t s0, ns;
@@ -86,6 +86,7 @@ struct PlanSample {
     int pad;
     double reach_r, reach_z;
     const int32_t *sstart, *pstart;
+    const double *srec;  // shape records (block bounding boxes)
     int64_t cell0, ncells;
     int64_t t0, nt;
 };
@@ -162,9 +163,12 @@ __device__ __forceinline__ void axis_class(double lo, double hi, double L, doubl
 // runs within reach of the block, merged while the periodic shift class and
 // the jackknife slab stay the same, plus the column's anomalous tail.  For
 // each range calls emit(local key, p_begin, p_end, rp, shift byte) with
-// local key = R_s * K1 + R_p.
+// local key = R_s * K1 + R_p.  Shift byte: 2 bits per axis (a, b, l) for the
+// constant shift class, bit 6 the per-pair wrap.  Reach is tested from the
+// bounding box bb of the block's shapes, not from its cells, so a column or
+// slab that no shape of the block can reach is left out.
 template <class F>
-__device__ __forceinline__ void plan_ranges(const PlanSample &P, const Block &B, int so, int K1,
+__device__ __forceinline__ void plan_ranges(const PlanSample &P, const Block &B, const double *bb, int so, int K1,
                                             F &&emit) {
     const Grid &g = P.g;
     int ox = so / P.nxw, oy = so % P.nxw;
@@ -183,22 +187,15 @@ __device__ __forceinline__ void plan_ranges(const PlanSample &P, const Block &B,
     double m = g.margin;
     double da_min, db_min;
     int ca, cb;
-    axis_class(ia * g.cw - (jx + 1) * g.cw - 2 * m, (ia + 1) * g.cw - jx * g.cw + 2 * m, g.L, g.half,
-               P.periodic, da_min, ca);
-    axis_class(ib * g.cw - (jy + 1) * g.cw - 2 * m, (ib + 1) * g.cw - jy * g.cw + 2 * m, g.L, g.half,
-               P.periodic, db_min, cb);
+    // separations between the block's shapes (their bounding box bb) and the
+    // target column's positions
+    axis_class(bb[0] - (jx + 1) * g.cw - 2 * m, bb[1] - jx * g.cw + 2 * m, g.L, g.half, P.periodic, da_min, ca);
+    axis_class(bb[2] - (jy + 1) * g.cw - 2 * m, bb[3] - jy * g.cw + 2 * m, g.L, g.half, P.periodic, db_min, cb);
     double rr = P.reach_r + 4 * m;
     double dxy2 = da_min * da_min + db_min * db_min;
     if (dxy2 >= rr * rr) return;
-    double zlo, zhi;
-    bool tail = B.grp >= P.ng;
-    if (tail) {
-        zlo = 0.0;
-        zhi = g.L;
-    } else {
-        zlo = (double)(B.grp * P.G) * g.hz;
-        zhi = (double)((B.grp + 1) * P.G) * g.hz;
-    }
+    // line-of-sight extent of the block's shapes (a tail block's shapes may lie anywhere)
+    const double zlo = bb[4], zhi = bb[5];
     int col = jx * g.nc + jy;
     int64_t base = (int64_t)col * (g.nz + 1);
     const int kr = B.rs * K1;
@@ -251,6 +248,7 @@ struct PlanLaunch {
     int K1;                   // regions + 1 of the largest sample's key space (keys per sample K1^2)
     const int32_t *boff;      // block offset of every cell (exclusive scan of the block counts)
     const Block *blocks;
+    const double *bbox;       // 6 doubles per block: bounding box of its shapes
     unsigned long long *key_count;   // count pass
     unsigned long long *key_cursor;  // emit pass
     const int64_t *key_off;
@@ -308,7 +306,7 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan_items(PlanLaunch A) {
         }
         if (!A.emit || local) {
             if (active)
-                plan_ranges(P, B, so, A.K1, [&](int k, int pb, int pe, int, uint8_t) {
+                plan_ranges(P, B, A.bbox + 6 * (int64_t)blk, so, A.K1, [&](int k, int pb, int pe, int, uint8_t) {
                     int n = (pe - pb + pmax - 1) / pmax;
                     if (local)
                         atomicAdd(&lcnt[k], n);
@@ -330,7 +328,7 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan_items(PlanLaunch A) {
             __syncthreads();
         }
         if (!A.emit) continue;
-        if (active) plan_ranges(P, B, so, A.K1, [&](int k, int pb, int pe, int rp, uint8_t sh) {
+        if (active) plan_ranges(P, B, A.bbox + 6 * (int64_t)blk, so, A.K1, [&](int k, int pb, int pe, int rp, uint8_t sh) {
             int n = (pe - pb + pmax - 1) / pmax;
             int64_t slot = A.key_off[kb + k] +
                            (local ? lbase[k] + (int64_t)atomicAdd(&lcnt[k], n)
@@ -368,7 +366,7 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan_items(PlanLaunch A) {
 // blocks of at most bsz shapes (count), then the blocks themselves at the
 // scanned offsets (emit).  cell0[S + 1] locates each sample's cells.
 __global__ void k_blocks(const PlanSample *ps, const int64_t *cell0, int S, int bsz,
-                         int32_t *nb, const int32_t *boff, Block *blocks) {
+                         int32_t *nb, const int32_t *boff, Block *blocks, double *bbox) {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= cell0[S]) return;
     const int s = find_sample(cell0, S, c);
@@ -401,6 +399,22 @@ __global__ void k_blocks(const PlanSample *ps, const int64_t *cell0, int S, int 
         B.sample = s;
         B.rs = rs;
         blocks[o + b] = B;
+        // bounding box {a_lo, a_hi, b_lo, b_hi, l_lo, l_hi} of the block's shapes
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (int j = 0; j < B.ns; ++j) {
+            const double *r = P.srec + 8 * (int64_t)(B.s0 + j);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                lo[k] = fmin(lo[k], r[k]);
+                hi[k] = fmax(hi[k], r[k]);
+            }
+        }
+        double *bb = bbox + 6 * (int64_t)(o + b);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            bb[2 * k] = lo[k];
+            bb[2 * k + 1] = hi[k];
+        }
     }
 }
 
@@ -458,8 +472,10 @@ struct PairArgs {
     const double *T, *X;         // exact thresholds (global copy)
     double T0, Tn, X0, Xn;       // domain bounds (rppi: s and pi; rmu: r^2)
     float lgs, lgc;              // r bin guess: log2(v) * lgs + lgc
-    float xs, xc;                // pi: pi * xs + xc; mu: (mu + 1) * xs
-    float dr, d2;                // guard bands: exact path when |guess - nearest integer| < d
+    float xs;                    // mu: (mu + 1) * xs
+    double xs64, xc64;           // pi: pi * xs64 + xc64
+    float dr;                    // guard bands: exact path when |guess - nearest integer| < d
+    float d2;
     int scan;                    // 1: the exact path scans every threshold (guard band of a whole bin)
     const SampleDev *samples;
     int hist_stride;             // bytes per wave histogram
@@ -469,6 +485,7 @@ struct PairArgs {
     unsigned long long *rechecks;
     // wall-clock span of the launch's busy waves: [0] max of ~start, [1] max of end
     unsigned long long *clock;
+    unsigned long long *iterations;  // wave-iterations (a 64-lane position batch against one shape)
 };
 
 // Wave histogram layout (C = 2^copies_log2 interleaved copies of B bins;
@@ -563,6 +580,12 @@ __device__ __forceinline__ int edge_near(const double *E, int n, double v, float
     if (e >= n) return n - 1;
     return e - 1 + (E[e] <= v ? 1 : 0);
 }
+__device__ __forceinline__ int edge_near(const double *E, int n, double v, double g) {
+    const int e = (int)rint(g);
+    if (e <= 0) return 0;
+    if (e >= n) return n - 1;
+    return e - 1 + (E[e] <= v ? 1 : 0);
+}
 
 #define MIA_CONST __attribute__((address_space(4)))
 
@@ -621,8 +644,8 @@ __device__ __forceinline__ void one_pair(const PairArgs &P, const SampleView &S,
                                          double pa, double pb, double pl, double pw, double qa,
                                          double qb, double ql, double qax, double qay, double qew,
                                          double qw, double q2ew, double sha, double shb, double shl,
-                                         char *hist, const double *T, const double *X, int copy,
-                                         uint32_t &nre, Pend &pd, float vlgc, float vxc) {
+                                         char *hist, const double *T, const double *X,
+                                         uint32_t &nre, Pend &pd, float vlgc, double vxc) {
     const int n_r = P.n_r, n_2 = P.n_2;
     // the reference's own FP64 separation (measure_w_box_jk.py:159-162)
     double da = qa - pa, db = qb - pb, dl = ql - pl;
@@ -651,18 +674,24 @@ __device__ __forceinline__ void one_pair(const PairArgs &P, const SampleView &S,
         // within the guard band of a bin edge (every edge, the domain edges
         // included, so no clamping is needed)
         float rv32 = (float)rv;
-        float dlf = (float)dl;
         float gr = __fmaf_rn(__log2f(rv32), P.lgs, vlgc);
-        float g2;
-        if (MODE == MIA_RPPI)
-            g2 = __fmaf_rn(dlf, P.xs, vxc);
-        else
-            g2 = __fmaf_rn(dlf, __builtin_amdgcn_rsqf(rv32), 1.0f) * P.xs;
+        // pi: an FP64 guess (one FMA; its error is ~1e-14 bins, so a 1e-9 guard
+        // band); mu = pi / r: an FP32 guess from rsqrt
+        double g2;
+        bool near2;
+        if (MODE == MIA_RPPI) {
+            g2 = fma(dl, P.xs64, vxc);
+            near2 = fabs(g2 - rint(g2)) < P.d2;
+        } else {
+            float g2f = __fmaf_rn((float)dl, __builtin_amdgcn_rsqf(rv32), 1.0f) * P.xs;
+            near2 = fabsf(g2f - rintf(g2f)) < P.d2;
+            g2 = g2f;
+        }
         // truncation is floor for the non-negative guesses; a guess within
         // the guard band of an integer (an edge, or a slightly negative
         // guess at the lower domain edge) takes the exact path
         int ir = (int)gr, i2 = (int)g2;
-        bool near = ((int)(fabsf(gr - rintf(gr)) < P.dr) | (int)(fabsf(g2 - rintf(g2)) < P.d2)) != 0;
+        bool near = ((int)(fabsf(gr - rintf(gr)) < P.dr) | (int)near2) != 0;
         if (near) {
             if (P.scan) {
                 ir = count_edges(T, n_r, rv);
@@ -693,7 +722,7 @@ __device__ __forceinline__ void one_pair(const PairArgs &P, const SampleView &S,
         double u = q2ew * t;
         double ep = fma(u * t, inv, -qew);
         double ex = (u * fabs(c)) * inv;
-        if (c * c < s * kSin2Exact) {
+        if (fabs(c) < fabs(t) * kSinExact) {
             // nearly parallel: the reference's d may round outside [-1, 1],
             // where arccos gives NaN and the pair adds no e+ (measure_w_box_jk.py:170-175)
             double rp = sqrt(s);
@@ -708,7 +737,7 @@ __device__ __forceinline__ void one_pair(const PairArgs &P, const SampleView &S,
             ep = pw * ep;
             ex = pw * ex;
         }
-        int b = ((__mul24(ir, n_2) + i2) << P.copies_log2) | copy;
+        int b = __mul24(ir, n_2) + i2;
         double dd = WEIGHTED ? pw * qw : 0.0;
         double vr = VAR ? ep * ep : 0.0;
         // a new slot flushes the pending run; the run then restarts from this
@@ -730,22 +759,23 @@ template <int MODE, int SH, bool WEIGHTED, bool VAR>
 __device__ __forceinline__ void run_item(const PairArgs &P, const SampleView &S, int p0, int np,
                                          int shifts, ShapePtr shp, int ns, char *hist,
                                          const double *T, const double *X, int lane,
-                                         uint32_t &nre) {
+                                         uint32_t &nre, uint64_t &iterations) {
     const double L = S.L;
     const double sha = SH == SH_CONST ? shift_val(shifts & 3, L) : 0.0;
     const double shb = SH == SH_CONST ? shift_val((shifts >> 2) & 3, L) : 0.0;
     const double shl = SH == SH_CONST ? shift_val((shifts >> 4) & 3, L) : 0.0;
-    const int copy = lane & ((1 << P.copies_log2) - 1);
     // the additive constants of the FP32 bin guesses, held in VGPRs for the
     // whole item (a VOP3 FMA reads one scalar operand; the other would be
     // copied into a VGPR on every pair)
-    float vlgc, vxc;
+    float vlgc;
+    double vxc;
     asm volatile("v_mov_b32 %0, %1" : "=v"(vlgc) : "s"(P.lgc));
-    asm volatile("v_mov_b32 %0, %1" : "=v"(vxc) : "s"(P.xc));
+    asm volatile("v_mov_b64 %0, %1" : "=v"(vxc) : "s"(P.xc64));
     // Strided lane -> position map: lane l of batch b takes position b + l*nb,
     // so one batch spans the whole line-of-sight range of the item and its
     // pairs spread over more (r, pi) bins (fewer same-address LDS atomics).
     const int nb = (np + kWave - 1) / kWave;
+    iterations += (uint64_t)nb * ns;
     Pend pd{-1, 0, 0, 0, 0, 0};
     for (int bt = 0; bt < nb; ++bt) {
         int pi0 = bt + lane * nb;
@@ -763,12 +793,12 @@ __device__ __forceinline__ void run_item(const PairArgs &P, const SampleView &S,
             const double b0 = r1[0], b1 = r1[1], b2 = r1[2], b3 = r1[3], b4 = r1[4], b5 = r1[5],
                          b6 = r1[6], b7 = r1[7];
             one_pair<MODE, SH, WEIGHTED, VAR>(P, S, v0, pa0, pb0, pl0, pw0, qa, qb, ql, qax, qay, qew, qw,
-                                         q2ew, sha, shb, shl, hist, T, X, copy, nre, pd, vlgc, vxc);
+                                         q2ew, sha, shb, shl, hist, T, X, nre, pd, vlgc, vxc);
             ShapePtr r2 = shp + 8 * (j + 2 < ns ? j + 2 : j + 1);
             const double c0 = r2[0], c1 = r2[1], c2 = r2[2], c3 = r2[3], c4 = r2[4], c5 = r2[5],
                          c6 = r2[6], c7 = r2[7];
             one_pair<MODE, SH, WEIGHTED, VAR>(P, S, v0, pa0, pb0, pl0, pw0, b0, b1, b2, b3, b4, b5, b6,
-                                         b7, sha, shb, shl, hist, T, X, copy, nre, pd, vlgc, vxc);
+                                         b7, sha, shb, shl, hist, T, X, nre, pd, vlgc, vxc);
             qa = c0;
             qb = c1;
             ql = c2;
@@ -780,7 +810,7 @@ __device__ __forceinline__ void run_item(const PairArgs &P, const SampleView &S,
         }
         if (j < ns)
             one_pair<MODE, SH, WEIGHTED, VAR>(P, S, v0, pa0, pb0, pl0, pw0, qa, qb, ql, qax, qay, qew, qw,
-                                         q2ew, sha, shb, shl, hist, T, X, copy, nre, pd, vlgc, vxc);
+                                         q2ew, sha, shb, shl, hist, T, X, nre, pd, vlgc, vxc);
     }
     if (pd.b >= 0) hist_add<WEIGHTED, VAR>(P, hist, pd.b, pd.c, pd.sp, pd.sx, pd.dd, pd.vr);
 }
@@ -817,6 +847,7 @@ __global__ __launch_bounds__(kThreads) void k_pairs(PairArgs P) {
     const SampleDev *Sp = nullptr;
     SampleView S{};
     uint32_t nre = 0;
+    uint64_t iterations = 0;  // wave-uniform
     do {
         for (int64_t i = it0; i < it1; ++i) {
             const MIA_CONST int32_t *ip = items + 4 * i;
@@ -839,17 +870,23 @@ __global__ __launch_bounds__(kThreads) void k_pairs(PairArgs P) {
             }
             ShapePtr shp = (ShapePtr)(S.srec + 8 * (int64_t)s0);
             if (shifts & 64)
-                run_item<MODE, SH_WRAP, WEIGHTED, VAR>(P, S, p0, np, shifts, shp, ns, hist, T, X, lane, nre);
+                run_item<MODE, SH_WRAP, WEIGHTED, VAR>(P, S, p0, np, shifts, shp, ns, hist, T, X, lane, nre,
+                                                       iterations);
             else if (shifts & 63)
-                run_item<MODE, SH_CONST, WEIGHTED, VAR>(P, S, p0, np, shifts, shp, ns, hist, T, X, lane, nre);
+                run_item<MODE, SH_CONST, WEIGHTED, VAR>(P, S, p0, np, shifts, shp, ns, hist, T, X, lane, nre,
+                                                        iterations);
             else
-                run_item<MODE, SH_NONE, WEIGHTED, VAR>(P, S, p0, np, shifts, shp, ns, hist, T, X, lane, nre);
+                run_item<MODE, SH_NONE, WEIGHTED, VAR>(P, S, p0, np, shifts, shp, ns, hist, T, X, lane, nre,
+                                                       iterations);
         }
     } while (sch.next(P.queue, lane, it0, it1));
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     if (Sp) flush_hist(P, hist, *Sp, cur_rs, cur_rp, lane);
     if (nre) atomicAdd(P.rechecks, (unsigned long long)nre);
-    if (lane == 0) atomicMax(&P.clock[1], (unsigned long long)wall_clock64());
+    if (lane == 0) {
+        if (iterations) atomicAdd(P.iterations, (unsigned long long)iterations);
+        atomicMax(&P.clock[1], (unsigned long long)wall_clock64());
+    }
 }
 
 template <int MODE, bool WEIGHTED, bool VAR>
@@ -923,22 +960,27 @@ PairConfig pair_config(const mia_bins &bins, bool any_w, bool any_var, const mia
     // (+ 2u n_r for a pass's shifted constant -l0 lgs - r0)
     double err_r = (4 * u * std::fmax(std::fabs(l0), std::fabs(ln)) + 4 * u) * lgs + 4 * u * n_r;
     double err_2;
+    double d2;
     if (bins.mode == MIA_RPPI) {
         A.X0 = bins.x2_edges[0];
         A.Xn = bins.x2_edges[n_2];
-        double xs = n_2 / (bins.x2_edges[n_2] - bins.x2_edges[0]);
-        A.xs = (float)xs;
-        A.xc = (float)(-bins.x2_edges[0] * xs);
-        double mx = std::fmax(std::fabs(bins.x2_edges[0]), std::fabs(bins.x2_edges[n_2]));
-        err_2 = 4 * u * mx * xs + 2 * u * n_2;
+        // FP64 guess fma(pi, xs, xc): a few FP64 ulps of |pi| xs + n_2 bins,
+        // far inside the 1e-9-bin guard band
+        const double xs = n_2 / (bins.x2_edges[n_2] - bins.x2_edges[0]);
+        A.xs64 = xs;
+        A.xc64 = -bins.x2_edges[0] * xs;
+        const double mx = std::fmax(std::fabs(bins.x2_edges[0]), std::fabs(bins.x2_edges[n_2]));
+        const double e64 = std::ldexp(1.0, -52);
+        err_2 = 8 * e64 * (mx * xs + n_2);
+        d2 = std::fmax(1e-9, 20 * err_2);
     } else {
         A.X0 = 0;
         A.Xn = 0;
         A.xs = (float)(n_2 / 2.0);
-        A.xc = 0.0f;
         err_2 = 6 * u * n_2;
+        d2 = std::fmax(1e-4, 20 * err_2);
     }
-    double dr = std::fmax(1e-4, 20 * err_r), d2 = std::fmax(1e-4, 20 * err_2);
+    double dr = std::fmax(1e-4, 20 * err_r);
     // pathological binnings (guard band > 5% of a bin): every pair takes the exact path
     bool exact_only = dr > 0.05 || d2 > 0.05;
     A.dr = exact_only ? 1.0f : (float)dr;
@@ -995,17 +1037,10 @@ PairConfig pair_config(const mia_bins &bins, bool any_w, bool any_var, const mia
     const int Bp = nrp * n_2;     // bins of the largest pass
     A.nbins_out = B;
     A.bin_off = 0;
-    // histogram copies (default 1: the pending-slot registers already absorb
-    // most collisions, and a single copy leaves LDS for 8 waves per SIMD);
-    // more copies only while LDS still holds ~5 waves per SIMD, at most 4
-    int copies_log2 = 0;
-    {
-        size_t per_copy = (size_t)Bp * (4 + 16 + (any_w ? 8 : 0) + (any_var ? 8 : 0));
-        int want = tuning && tuning->hist_copies > 0 ? tuning->hist_copies : 1;
-        while ((1 << (copies_log2 + 1)) <= want && (1 << (copies_log2 + 1)) <= 4 &&
-               per_copy * (2u << copies_log2) * 20 <= 160 * 1024)
-            ++copies_log2;
-    }
+    // one histogram copy per wave: the pending-slot registers already absorb
+    // most same-address collisions, and one copy leaves LDS for 8 waves per
+    // SIMD (2 copies measured 20 % slower); mia_tuning.hist_copies is ignored
+    const int copies_log2 = 0;
     A.copies_log2 = copies_log2;
     const int BC = Bp << copies_log2;
     int off = 4 * BC;
@@ -1072,6 +1107,7 @@ struct CountPlan {
     int K1 = 1, plan_local = 0, bsz = kMaxBlockShapes;
     int64_t ncells = 0, nthreads = 0, nkeys = 0, fin_total = 0, item_cap = 0;
     int64_t acc_total = 0;           // accumulator words (u64 / f64) of all samples
+    double margin = 0;               // largest geometric slack of the samples' grids
     PairConfig pc;
     bool any_w = false, any_var = false;
     std::vector<double> T, X;        // thresholds (X padded to n_2 + 1)
@@ -1094,6 +1130,7 @@ struct CountPlan {
     int64_t *ko = nullptr;
     Item *items = nullptr;
     int64_t *cost = nullptr, *cpre = nullptr, *n_items = nullptr, *bounds = nullptr;
+    double *bbox = nullptr;
     unsigned long long *words = nullptr;   // [kCountReadback] read back after the count
     int *queue = nullptr;
     unsigned long long *acc = nullptr;
@@ -1171,6 +1208,7 @@ CountPlan *plan_count(Ctx &ctx, const std::vector<SortedSample> &ss, const mia_b
         p.G[si] = G;
         PlanSample &Q = p.ps[si];
         std::memset(&Q, 0, sizeof(Q));
+        p.margin = std::max(p.margin, g.margin);
         Q.g = g;
         Q.mode = bins.mode;
         Q.periodic = x.periodic;
@@ -1272,6 +1310,7 @@ void count_bind(CountPlan &p, Carve &c, const std::vector<SortedSample> &ss, con
     int64_t blk_total = 0;
     for (int si = 0; si < S; ++si) blk_total += p.ps[si].nt / (p.ps[si].nxw * p.ps[si].nxw);
     p.blocks = c.take<Block>(std::max<int64_t>(blk_total, 1));
+    p.bbox = c.take<double>(6 * std::max<int64_t>(blk_total, 1));
     p.ko = c.take<int64_t>(p.nkeys + 1);
     scan_carve<int64_t>(c, p.nkeys, p.scan_keys);
     p.items = c.take<Item>(p.item_cap);
@@ -1300,6 +1339,7 @@ void count_bind(CountPlan &p, Carve &c, const std::vector<SortedSample> &ss, con
         PlanSample &Q = p.ps[si];
         Q.sstart = x.sstart;
         Q.pstart = x.pstart;
+        Q.srec = x.srec;
         SampleDev &D = p.sd[si];
         std::memset(&D, 0, sizeof(D));
         D.pa = x.pa; D.pb = x.pb; D.pl = x.pl; D.pw = x.pw;
@@ -1346,11 +1386,11 @@ void count_enqueue(const CountPlan &p, hipStream_t st, const hipEvent_t *ev) {
     fill<unsigned long long>(p.zero_begin, 0ull, (int64_t)p.zero_words, st);
     // shape blocks of every (column, slab group) cell
     hipLaunchKernelGGL(k_blocks, dim3(nblk(p.ncells, 256)), dim3(256), 0, st, ps, cell0, S, p.bsz, p.nb,
-                       (const int32_t *)nullptr, (Block *)nullptr);
+                       (const int32_t *)nullptr, (Block *)nullptr, (double *)nullptr);
     MIA_HIP(hipGetLastError());
     scan_ws<int32_t>(p.nb, p.boff, p.ncells, nullptr, p.scan_nb, st);
     hipLaunchKernelGGL(k_blocks, dim3(nblk(p.ncells, 256)), dim3(256), 0, st, ps, cell0, S, p.bsz, p.nb,
-                       (const int32_t *)p.boff, p.blocks);
+                       (const int32_t *)p.boff, p.blocks, p.bbox);
     MIA_HIP(hipGetLastError());
     // work items, counting-sorted by (sample, R_s, R_p)
     PlanLaunch L{};
@@ -1360,6 +1400,7 @@ void count_enqueue(const CountPlan &p, hipStream_t st, const hipEvent_t *ev) {
     L.K1 = p.K1;
     L.boff = p.boff;
     L.blocks = p.blocks;
+    L.bbox = p.bbox;
     L.key_count = p.kc;
     L.key_cursor = p.kcur;
     L.key_off = p.ko;
@@ -1396,6 +1437,7 @@ void count_enqueue(const CountPlan &p, hipStream_t st, const hipEvent_t *ev) {
     A.samples = (const SampleDev *)(pd + p.off_sd);
     A.rechecks = p.words;
     A.clock = p.words + 4;
+    A.iterations = p.words + 6;
     const int n_r = p.pc.A.n_r, n_2 = p.pc.A.n_2;
     for (int pass = 0; pass < p.pc.npass; ++pass) {
         const int r0 = pass * p.pc.nrp, r1 = std::min(n_r, r0 + p.pc.nrp);

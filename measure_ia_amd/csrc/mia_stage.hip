@@ -537,6 +537,7 @@ void fill_stats(mia_stats *stats, const RunOut &r, const CountPlan &cp, const Gr
     stats->ms_total = r.ms_total;
     stats->rechecks = (int64_t)r.words[0];
     stats->candidates = (int64_t)r.words[1];
+    stats->wave_iterations = (int64_t)r.words[6];
     stats->items = (int64_t)r.words[3];
     stats->columns_per_side = g0.nc;
     stats->slabs_per_side = g0.nz;
