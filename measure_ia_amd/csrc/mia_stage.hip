@@ -42,6 +42,7 @@ constexpr int kStageThreads = 256;
 constexpr int kStageItems = 4;                        // objects per thread
 constexpr int kStageWg = kStageThreads * kStageItems; // objects per workgroup
 constexpr int kLdsRegions = 4096;                     // region stats in LDS up to this many regions
+constexpr int kGroupRounds = 8;                       // cell-key groups formed per wave and object
 
 // One segment: the positions (kind 0) or the shapes (kind 1) of one sample.
 struct PrepSeg {
@@ -139,24 +140,44 @@ __global__ __launch_bounds__(kStageThreads) void k_stage_classify(const PrepSeg 
         const int cb = stage_axis_cell(x[g.ax_b], anom ? -1 : ji[g.ax_b], g.jk_sub, g.ncj, g.cw, g.nc);
         const int cl = stage_axis_cell(x[g.los], anom ? -1 : ji[g.los], g.jk_sub, g.nzj, g.hz, g.nz);
         const int64_t key = G.key0 + (int64_t)(ca * g.nc + cb) * (g.nz + 1) + (anom ? g.nz : cl);
-        const int rank = atomicAdd(&G.cnt[key], 1);
-        G.kr[G.obj0 + i] = make_int2((int)key, rank);
-        if (K > 0) {
-            double wi = 0.0, q = 0.0;
-            if (G.kind == 1) {
-                wi = G.w ? G.w[i] : 1.0;
-                const double ei = G.e[i];
-                q = wi * (1 - ei * ei / 2.0);
+        // lanes of the wave that share a cell key (neighbouring objects of the
+        // input often do) take their ranks from one atomic of a leader lane:
+        // up to kGroupRounds groups, the rest one atomic each
+        const int lane = threadIdx.x & (kWave - 1);
+        const int k32 = (int)key;
+        int lead = lane, sub = 0, inc = 1;
+        uint64_t rem = __ballot(1);
+        for (int round = 0; round < kGroupRounds && rem; ++round) {
+            const int l = __builtin_ffsll((long long)rem) - 1;
+            const int k0 = __builtin_amdgcn_readlane(k32, l);
+            const uint64_t m = __ballot(k32 == k0) & rem;
+            if ((m >> lane) & 1) {
+                lead = l;
+                sub = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                inc = lane == l ? __popcll(m) : 0;
             }
-            if (lds) {
-                atomicAdd(&lcnt[region], 1u);
-                if (G.kind == 1) {
+            rem &= ~m;
+        }
+        int base = 0;
+        if (inc > 0) base = atomicAdd(&G.cnt[key], inc);
+        const int rank = __shfl(base, lead) + sub;
+        G.kr[G.obj0 + i] = make_int2(k32, rank);
+        if (K > 0) {
+            // a group shares its cell, hence its region: its leader counts it
+            if (inc > 0) {
+                if (lds)
+                    atomicAdd(&lcnt[region], (unsigned int)inc);
+                else
+                    atomicAdd(&G.rcnt[G.kind * K + region], (unsigned long long)inc);
+            }
+            if (G.kind == 1) {
+                const double wi = G.w ? G.w[i] : 1.0;
+                const double ei = G.e[i];
+                const double q = wi * (1 - ei * ei / 2.0);
+                if (lds) {
                     atomicAdd(&lstat[region], wi);
                     atomicAdd(&lstat[K + region], q);
-                }
-            } else {
-                atomicAdd(&G.rcnt[G.kind * K + region], 1ull);
-                if (G.kind == 1) {
+                } else {
                     atomicAdd(&G.rsum[region], wi);
                     atomicAdd(&G.rsum[K + region], q);
                 }
