@@ -57,7 +57,12 @@ sys.path.insert(0, ROOT)
 PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 vector (= FP32 without packed instructions)
 PEAK_FP32_PACKED_TFLOPS = 157.3
 PEAK_HBM_GBPS = 8000.0       # MI355X HBM3E
-SORT_BYTES = (72, 112)       # algorithmic bytes per position / per shape of the counting sort
+# algorithmic bytes of the counting sort per object: classify reads the
+# coordinates (24 B; shapes also e, 8 B) and writes (key, rank) (8 B); scatter
+# reads (key, rank), the key's start (4 B) and the coordinates again and writes
+# the 32-byte position record or the 64-byte shape record (axis 16 B, e 8 B
+# read again); a weight array adds 8 B to each read of the object.
+SORT_BYTES = {"pos": 100, "shp": 164, "w_pos": 8, "w_shp": 16}
 FLOP_PER_PAIR = {"rppi": 28, "rmu": 32}   # algorithmic FLOP per in-domain pair
 
 W_BINS = dict(mode="rppi", r=(0.1, 20.0), nr=10, n2=20, pi_max=60.0, jk=3)
@@ -277,6 +282,45 @@ def run_lightcone(a, w):
     print(json.dumps(line))
 
 
+def weighted_prep(p):
+    return not (np.all(p.weight == 1.0) and np.all(p.weight_shape == 1.0))
+
+
+def sort_model_bytes(p):
+    w = weighted_prep(p)
+    return ((SORT_BYTES["pos"] + w * SORT_BYTES["w_pos"]) * p.n_pos +
+            (SORT_BYTES["shp"] + w * SORT_BYTES["w_shp"]) * p.n_shape)
+
+
+def sort_only(a, eng, samples, preps, th, batch):
+    """The cell sort alone, --steps times after --warmup: a batch through
+    mia_count_batch with plain launches (the pair count runs too; its stats
+    give the sort's own event time), a single sample through
+    mia_prepare_sample (timed on the host around a synchronised call)."""
+    import torch
+    times = []
+    for k in range(a.warmup + a.steps):
+        if batch:
+            ms = eng.count_batch(samples, th, tuning={"graph": -1})[0].stats["ms_prepare"]
+        else:
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            h = eng.prepare(samples[0], th)
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t0) * 1e3
+            h.release()
+        if k >= a.warmup:
+            times.append(ms)
+        log(f"sort {k}: {ms:.3f} ms")
+    nbytes = sum(sort_model_bytes(p) for p in preps)
+    ms = float(np.median(times))
+    print(json.dumps({"sort_only": a.workload, "ms_median": ms, "runs_ms": times, "model_bytes": nbytes,
+                      "model_GBps": nbytes / (ms / 1e3) / 1e9,
+                      "timing": "event time of the sort inside the count" if batch else
+                                "host time around mia_prepare_sample + synchronise (includes the launch "
+                                "and parameter upload)"}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -286,6 +330,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain GPU comparator")
     ap.add_argument("--no-e2e", action="store_true", help="skip the end-to-end measure_xi_w call")
+    ap.add_argument("--sort-only", action="store_true",
+                    help="only run the cell sort of the workload's samples --steps times (single samples "
+                         "through mia_prepare_sample, batches through mia_count_batch with plain launches) "
+                         "and print its timings; for profiling the sort kernels")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three 512 MiB device copies (the reference kernel for the FETCH_SIZE / "
                          "WRITE_SIZE scale factors) and exit")
@@ -336,6 +384,10 @@ def main():
     # inputs resident in HBM before the timed region
     samples = [dev_sample(p) for p in preps]
     torch.cuda.synchronize()
+
+    if a.sort_only:
+        sort_only(a, eng, samples, preps, th, batch)
+        return
 
     def step():
         if batch:
@@ -455,11 +507,13 @@ def main():
     st = res[0].stats
     # the counting sort (region ids, cell keys, scan, scatter of every sample) against HBM
     sort_ms = float(np.mean([s["ms_prepare"] for s in ev_stats]))
-    sort_bytes = sum(SORT_BYTES[0] * p.n_pos + SORT_BYTES[1] * p.n_shape for p in preps)
+    sort_bytes = sum(sort_model_bytes(p) for p in preps)
     sort_gbps = sort_bytes / (sort_ms / 1e3) / 1e9
     cell_sort = {"bound": "hbm", "achieved": sort_gbps, "peak": PEAK_HBM_GBPS, "unit": "GB/s",
                  "frac": sort_gbps / PEAK_HBM_GBPS, "ms": sort_ms, "bytes": sort_bytes,
-                 "model": "72 B per position + 112 B per shape (read, key, scatter)"}
+                 "model": "100 B per position + 164 B per shape (+8 / +16 B when weighted): classify reads "
+                          "coordinates, writes (key, rank); scatter reads them again and writes the records; "
+                          "the time covers the zero fill, classify, two scans and the scatter"}
     line = {
         "metric": "pairs/s & s per wgg+wg+ (1M gal, 27 JK): batched-FP32 vs plain GPU",
         "value": value, "unit": "pairs/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,

@@ -243,7 +243,7 @@ struct SortedSample {
     int K = 0;                        // jackknife regions (0: none)
     int periodic = 1;
     int weighted = 0;                 // a weight array was given
-    const double *pa = nullptr, *pb = nullptr, *pl = nullptr, *pw = nullptr;  // positions (a, b, los, w)
+    const double *prec = nullptr;     // 4 doubles per position {a, b, l, w}, sorted order
     const double *srec = nullptr;     // 8 doubles per shape {a, b, l, ax, ay, e*w, w, 2*e*w}
     const int32_t *pstart = nullptr, *sstart = nullptr;  // keys + 1 start offsets into the arrays above
     const unsigned long long *rcnt = nullptr;  // [2K] per region: positions, shapes

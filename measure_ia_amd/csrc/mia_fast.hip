@@ -62,7 +62,7 @@ static_assert(sizeof(Block) == 24, "Block layout");
 
 struct SampleDev {
     // sorted sample
-    const double *pa, *pb, *pl, *pw;
+    const double *prec;          // 4 doubles per position {a, b, l, w}, sorted order
     const double *srec;          // 8 doubles per shape, sorted order
     const int32_t *pstart;
     double L, half;
@@ -592,7 +592,7 @@ __device__ __forceinline__ int edge_near(const double *E, int n, double v, doubl
 // Per-sample values the pair loop needs, read once per item through the
 // scalar cache (they are wave-uniform and end up in SGPRs).
 struct SampleView {
-    const double *pa, *pb, *pl, *pw;
+    const double *prec;
     const double *srec;
     double L, half;
     int los;
@@ -601,10 +601,7 @@ struct SampleView {
 __device__ __forceinline__ SampleView load_view(const SampleDev *sd) {
     const MIA_CONST SampleDev *c = (const MIA_CONST SampleDev *)sd;
     SampleView v;
-    v.pa = c->pa;
-    v.pb = c->pb;
-    v.pl = c->pl;
-    v.pw = c->pw;
+    v.prec = c->prec;
     v.srec = c->srec;
     v.L = c->L;
     v.half = c->half;
@@ -781,8 +778,10 @@ __device__ __forceinline__ void run_item(const PairArgs &P, const SampleView &S,
         int pi0 = bt + lane * nb;
         bool v0 = pi0 < np;
         int p0i = p0 + (v0 ? pi0 : 0);
-        double pa0 = S.pa[p0i], pb0 = S.pb[p0i], pl0 = S.pl[p0i];
-        double pw0 = WEIGHTED ? S.pw[p0i] : 1.0;
+        const double2 *pr = (const double2 *)(S.prec + 4 * (int64_t)p0i);
+        const double2 ab = pr[0], lw = pr[1];
+        double pa0 = ab.x, pb0 = ab.y, pl0 = lw.x;
+        double pw0 = WEIGHTED ? lw.y : 1.0;
         // two shapes per iteration, the record after them loaded ahead (scalar
         // loads while the current pairs are evaluated)
         double qa = shp[0], qb = shp[1], ql = shp[2], qax = shp[3], qay = shp[4], qew = shp[5],
@@ -1342,7 +1341,7 @@ void count_bind(CountPlan &p, Carve &c, const std::vector<SortedSample> &ss, con
         Q.srec = x.srec;
         SampleDev &D = p.sd[si];
         std::memset(&D, 0, sizeof(D));
-        D.pa = x.pa; D.pb = x.pb; D.pl = x.pl; D.pw = x.pw;
+        D.prec = x.prec;
         D.srec = x.srec;
         D.pstart = x.pstart;
         D.L = x.g.L;

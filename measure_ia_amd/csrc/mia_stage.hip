@@ -15,7 +15,7 @@
 //              weight sums, aggregated per workgroup in LDS.
 //   scan     : one exclusive scan per kind over the concatenated key counters
 //              of all samples, so every start is a global index.
-//   scatter  : one launch: positions to SoA (a, b, l, w), shapes to 64-byte
+//   scatter  : one launch: positions to 32-byte records {a, b, l, w}, shapes to 64-byte
 //              records {a, b, l, ax, ay, e*w, w, 2*e*w}.
 // Nothing is read back until the count has finished: sizes the device decides
 // stay on the device, and every buffer is carved from one workspace whose
@@ -58,7 +58,7 @@ struct PrepSeg {
     int2 *kr;                 // (global key, rank in cell) of every object of the kind
     unsigned long long *rcnt; // [2K] of the sample: positions, shapes per region
     double *rsum;             // [2K] of the sample: sum w_s, sum w_s (1 - e^2/2) per region
-    double *pa, *pb, *pl, *pw;
+    double *prec;
     double *srec;
     int *oor;                 // the sample's out-of-box flag
 };
@@ -98,8 +98,10 @@ __device__ __forceinline__ int stage_axis_cell(double x, int ji, double jk_sub, 
 __global__ __launch_bounds__(kStageThreads) void k_stage_classify(const PrepSeg *segs, const int64_t *wg0,
                                                                   int nseg, int lds_regions) {
     extern __shared__ double lstat[];  // [2K] sums, then [K] counts (u32)
-    const int j = find_seg(wg0, nseg, blockIdx.x);
-    const PrepSeg &G = segs[j];
+    // the segment is the workgroup's: a uniform index, so its fields are read
+    // once into scalar registers instead of by vector loads at every use
+    const int j = __builtin_amdgcn_readfirstlane(find_seg(wg0, nseg, blockIdx.x));
+    const PrepSeg G = segs[j];
     const Grid &g = G.g;
     const int K = G.K;
     const bool lds = K > 0 && K <= lds_regions;
@@ -114,72 +116,98 @@ __global__ __launch_bounds__(kStageThreads) void k_stage_classify(const PrepSeg 
     }
     bool bad = false;
     const int64_t base = (int64_t)(blockIdx.x - G.wg0) * kStageWg + threadIdx.x;
+    const int lane = threadIdx.x & (kWave - 1);
+    // 1: every item's coordinates -> region and cell key (all loads in flight
+    // together; a lane past the segment's end reads object 0 and counts nothing)
+    bool valid[kStageItems];
+    int key[kStageItems], region[kStageItems];
+    double xs[kStageItems][3], wq[kStageItems][2];
+    const bool stats = K > 0 && G.kind == 1;
 #pragma unroll
     for (int it = 0; it < kStageItems; ++it) {
-        const int64_t i = base + (int64_t)it * kStageThreads;
-        if (i >= G.n) break;
-        double x[3] = {G.xyz[3 * i], G.xyz[3 * i + 1], G.xyz[3 * i + 2]};
+        const int64_t i0 = base + (int64_t)it * kStageThreads;
+        valid[it] = i0 < G.n;
+        const int64_t i = valid[it] ? i0 : 0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) xs[it][c] = G.xyz[3 * i + c];
+        wq[it][0] = stats && G.w ? G.w[i] : 1.0;
+        wq[it][1] = stats ? G.e[i] : 0.0;
+    }
+#pragma unroll
+    for (int it = 0; it < kStageItems; ++it) {
+        double *x = xs[it];
         bool ok = true;
 #pragma unroll
         for (int c = 0; c < 3; ++c) ok = ok && (x[c] >= 0.0) && (x[c] <= g.L);
         if (!ok) {
-            bad = true;
+            bad = bad || valid[it];
 #pragma unroll
             for (int c = 0; c < 3; ++c) x[c] = 0.0;
         }
         int ji[3] = {-1, -1, -1};
         bool anom = false;
-        int region = 0;
+        region[it] = 0;
         if (g.jk > 0) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) ji[c] = strict_axis_index(x[c], g.jk_sub, g.jk);
             anom = ji[0] < 0 || ji[1] < 0 || ji[2] < 0;
-            if (!anom) region = ji[0] * g.jk * g.jk + ji[1] * g.jk + ji[2];
+            if (!anom) region[it] = ji[0] * g.jk * g.jk + ji[1] * g.jk + ji[2];
         }
         const int ca = stage_axis_cell(x[g.ax_a], anom ? -1 : ji[g.ax_a], g.jk_sub, g.ncj, g.cw, g.nc);
         const int cb = stage_axis_cell(x[g.ax_b], anom ? -1 : ji[g.ax_b], g.jk_sub, g.ncj, g.cw, g.nc);
         const int cl = stage_axis_cell(x[g.los], anom ? -1 : ji[g.los], g.jk_sub, g.nzj, g.hz, g.nz);
-        const int64_t key = G.key0 + (int64_t)(ca * g.nc + cb) * (g.nz + 1) + (anom ? g.nz : cl);
-        // lanes of the wave that share a cell key (neighbouring objects of the
-        // input often do) take their ranks from one atomic of a leader lane:
-        // up to kGroupRounds groups, the rest one atomic each
-        const int lane = threadIdx.x & (kWave - 1);
-        const int k32 = (int)key;
-        int lead = lane, sub = 0, inc = 1;
-        uint64_t rem = __ballot(1);
+        key[it] = (int)(G.key0 + (int64_t)(ca * g.nc + cb) * (g.nz + 1) + (anom ? g.nz : cl));
+        // per-region sums of shapes: w and w (1 - e^2/2)
+        const double ei = wq[it][1];
+        wq[it][1] = wq[it][0] * (1 - ei * ei / 2.0);
+    }
+    // 2: the key counters' atomics of all items, issued back to back.  Lanes
+    // of the wave that share a cell key (neighbouring objects of the input
+    // often do) take their ranks from one atomic of a leader lane: up to
+    // kGroupRounds groups per item, the rest one atomic each.
+    int lead[kStageItems], sub[kStageItems], inc[kStageItems], got[kStageItems];
+#pragma unroll
+    for (int it = 0; it < kStageItems; ++it) {
+        lead[it] = lane;
+        sub[it] = 0;
+        inc[it] = valid[it] ? 1 : 0;
+        uint64_t rem = __ballot(valid[it]);
         for (int round = 0; round < kGroupRounds && rem; ++round) {
             const int l = __builtin_ffsll((long long)rem) - 1;
-            const int k0 = __builtin_amdgcn_readlane(k32, l);
-            const uint64_t m = __ballot(k32 == k0) & rem;
+            const int k0 = __builtin_amdgcn_readlane(key[it], l);
+            const uint64_t m = __ballot(key[it] == k0) & rem;
             if ((m >> lane) & 1) {
-                lead = l;
-                sub = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-                inc = lane == l ? __popcll(m) : 0;
+                lead[it] = l;
+                sub[it] = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                inc[it] = lane == l ? __popcll(m) : 0;
             }
             rem &= ~m;
         }
-        int base = 0;
-        if (inc > 0) base = atomicAdd(&G.cnt[key], inc);
-        const int rank = __shfl(base, lead) + sub;
-        G.kr[G.obj0 + i] = make_int2(k32, rank);
+        got[it] = 0;
+        if (inc[it] > 0) got[it] = atomicAdd(&G.cnt[key[it]], inc[it]);
+    }
+    // 3: ranks, (key, rank) records and the per-region statistics
+#pragma unroll
+    for (int it = 0; it < kStageItems; ++it) {
+        const int rank = __shfl(got[it], lead[it]) + sub[it];
+        if (!valid[it]) continue;
+        G.kr[G.obj0 + base + (int64_t)it * kStageThreads] = make_int2(key[it], rank);
         if (K > 0) {
             // a group shares its cell, hence its region: its leader counts it
-            if (inc > 0) {
+            const int rg = region[it];
+            if (inc[it] > 0) {
                 if (lds)
-                    atomicAdd(&lcnt[region], (unsigned int)inc);
+                    atomicAdd(&lcnt[rg], (unsigned int)inc[it]);
                 else
-                    atomicAdd(&G.rcnt[G.kind * K + region], (unsigned long long)inc);
+                    atomicAdd(&G.rcnt[G.kind * K + rg], (unsigned long long)inc[it]);
             }
             if (G.kind == 1) {
-                const double wi = G.w ? G.w[i] : 1.0;
-                const double ei = G.e[i];
-                const double q = wi * (1 - ei * ei / 2.0);
                 if (lds) {
-                    atomicAdd(&lstat[region], wi);
-                    atomicAdd(&lstat[K + region], q);
+                    atomicAdd(&lstat[rg], wq[it][0]);
+                    atomicAdd(&lstat[K + rg], wq[it][1]);
                 } else {
-                    atomicAdd(&G.rsum[region], wi);
-                    atomicAdd(&G.rsum[K + region], q);
+                    atomicAdd(&G.rsum[rg], wq[it][0]);
+                    atomicAdd(&G.rsum[K + rg], wq[it][1]);
                 }
             }
         }
@@ -199,43 +227,72 @@ __global__ __launch_bounds__(kStageThreads) void k_stage_classify(const PrepSeg 
     }
 }
 
-// Counting-sort scatter to start[key] + rank.
+// Counting-sort scatter to start[key] + rank: all of a thread's loads are
+// issued before its stores (the records may alias the inputs as far as the
+// compiler knows, which would otherwise serialise the items).
 __global__ __launch_bounds__(kStageThreads) void k_stage_scatter(const PrepSeg *segs, const int64_t *wg0,
                                                                  int nseg) {
-    const int j = find_seg(wg0, nseg, blockIdx.x);
-    const PrepSeg &G = segs[j];
+    // the segment is the workgroup's: a uniform index, so its fields are read
+    // once into scalar registers instead of by vector loads at every use
+    const int j = __builtin_amdgcn_readfirstlane(find_seg(wg0, nseg, blockIdx.x));
+    const PrepSeg G = segs[j];
     const Grid &g = G.g;
     const int64_t base = (int64_t)(blockIdx.x - G.wg0) * kStageWg + threadIdx.x;
+    bool valid[kStageItems];
+    int64_t idx[kStageItems], d[kStageItems];
 #pragma unroll
     for (int it = 0; it < kStageItems; ++it) {
-        const int64_t i = base + (int64_t)it * kStageThreads;
-        if (i >= G.n) break;
-        const int2 kr = G.kr[G.obj0 + i];
-        const int64_t d = (int64_t)G.start[kr.x] + kr.y;
-        const double a = G.xyz[3 * i + g.ax_a], b = G.xyz[3 * i + g.ax_b], l = G.xyz[3 * i + g.los];
-        const double wi = G.w ? G.w[i] : 1.0;
-        if (G.kind == 0) {
-            G.pa[d] = a;
-            G.pb[d] = b;
-            G.pl[d] = l;
-            G.pw[d] = wi;
-            continue;
+        const int64_t i0 = base + (int64_t)it * kStageThreads;
+        valid[it] = i0 < G.n;
+        idx[it] = valid[it] ? i0 : 0;
+        const int2 kr = G.kr[G.obj0 + idx[it]];
+        d[it] = (int64_t)G.start[kr.x] + kr.y;
+    }
+    double a[kStageItems], b[kStageItems], l[kStageItems], wi[kStageItems];
+#pragma unroll
+    for (int it = 0; it < kStageItems; ++it) {
+        const int64_t i = idx[it];
+        a[it] = G.xyz[3 * i + g.ax_a];
+        b[it] = G.xyz[3 * i + g.ax_b];
+        l[it] = G.xyz[3 * i + g.los];
+        wi[it] = G.w ? G.w[i] : 1.0;
+    }
+    if (G.kind == 0) {
+#pragma unroll
+        for (int it = 0; it < kStageItems; ++it) {
+            if (!valid[it]) continue;
+            double2 *r = (double2 *)(G.prec + 4 * d[it]);
+            r[0] = make_double2(a[it], b[it]);
+            r[1] = make_double2(l[it], wi[it]);
         }
-        double ax = G.axis[2 * i], ay = G.axis[2 * i + 1], ei = G.e[i];
+        return;
+    }
+    double ax[kStageItems], ay[kStageItems], ei[kStageItems];
+#pragma unroll
+    for (int it = 0; it < kStageItems; ++it) {
+        const int64_t i = idx[it];
+        ax[it] = G.axis[2 * i];
+        ay[it] = G.axis[2 * i + 1];
+        ei[it] = G.e[i];
+    }
+#pragma unroll
+    for (int it = 0; it < kStageItems; ++it) {
+        if (!valid[it]) continue;
+        double qx = ax[it], qy = ay[it], qe = ei[it];
         // a shape whose axis or e is NaN (e.g. a zero axis vector normalised on
         // the host) adds e+ = ex = 0, as the reference's isnan rule does
         // (measure_w_box_jk.py:174-175): zero its axis and e in the record
-        if (!(ax == ax) || !(ay == ay) || !(ei == ei)) {
-            ax = 0.0;
-            ay = 0.0;
-            ei = 0.0;
+        if (!(qx == qx) || !(qy == qy) || !(qe == qe)) {
+            qx = 0.0;
+            qy = 0.0;
+            qe = 0.0;
         }
-        const double ew = ei * wi;
-        double2 *r = (double2 *)(G.srec + 8 * d);
-        r[0] = make_double2(a, b);
-        r[1] = make_double2(l, ax);
-        r[2] = make_double2(ay, ew);
-        r[3] = make_double2(wi, 2.0 * ew);
+        const double ew = qe * wi[it];
+        double2 *r = (double2 *)(G.srec + 8 * d[it]);
+        r[0] = make_double2(a[it], b[it]);
+        r[1] = make_double2(l[it], qx);
+        r[2] = make_double2(qy, ew);
+        r[3] = make_double2(wi[it], 2.0 * ew);
     }
 }
 
@@ -257,7 +314,7 @@ struct PrepPlan {
     int32_t *pcnt = nullptr, *scnt = nullptr, *pstart = nullptr, *sstart = nullptr;
     ScanBufs<int32_t> scan_p, scan_s;
     int2 *pkr = nullptr, *skr = nullptr;
-    double *pa = nullptr, *pb = nullptr, *pl = nullptr, *pw = nullptr, *srec = nullptr;
+    double *prec = nullptr, *srec = nullptr;
     unsigned long long *rcnt = nullptr;
     double *rsum = nullptr;
     int *oor = nullptr;
@@ -336,10 +393,7 @@ std::vector<SortedSample> prep_bind(PrepPlan &p, Carve &c, const mia_sample *sam
     scan_carve<int32_t>(c, p.keys, p.scan_s);
     p.pkr = c.take<int2>(p.npos);
     p.skr = c.take<int2>(p.nshp);
-    p.pa = c.take<double>(p.npos);
-    p.pb = c.take<double>(p.npos);
-    p.pl = c.take<double>(p.npos);
-    p.pw = c.take<double>(p.npos);
+    p.prec = c.take<double>(4 * p.npos);
     p.srec = c.take<double>(8 * p.nshp);
     std::vector<SortedSample> ss(S);
     for (int s = 0; s < S; ++s) {
@@ -353,10 +407,7 @@ std::vector<SortedSample> prep_bind(PrepPlan &p, Carve &c, const mia_sample *sam
         x.periodic = m.periodic;
         x.weighted = (m.pos_w != nullptr || m.shp_w != nullptr) ? 1 : 0;
         if (c.dry) continue;
-        x.pa = p.pa;
-        x.pb = p.pb;
-        x.pl = p.pl;
-        x.pw = p.pw;
+        x.prec = p.prec;
         x.srec = p.srec;
         x.pstart = p.pstart + p.key0[s];
         x.sstart = p.sstart + p.key0[s];
@@ -373,10 +424,7 @@ std::vector<SortedSample> prep_bind(PrepPlan &p, Carve &c, const mia_sample *sam
             G.kr = kind ? p.skr : p.pkr;
             G.rcnt = p.rcnt + p.rs0[s];
             G.rsum = p.rsum + p.rs0[s];
-            G.pa = p.pa;
-            G.pb = p.pb;
-            G.pl = p.pl;
-            G.pw = p.pw;
+            G.prec = p.prec;
             G.srec = p.srec;
             G.oor = p.oor + s;
         }
