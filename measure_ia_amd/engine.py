@@ -128,6 +128,17 @@ class Engine:
             raise ValueError(f"array shape {tuple(x.shape)} != {shape}")
         return x
 
+    def to_device(self, arrays: dict) -> dict:
+        """Device float64 copies of a sample's host arrays (for repeated counts);
+        weight arrays of unit weights become None (the unweighted path)."""
+        out = {}
+        for k, v in arrays.items():
+            if k in ("weight", "weight_shape") and _unit(v):
+                out[k] = None
+            else:
+                out[k] = self._dev(v)
+        return out
+
     @staticmethod
     def _bins(th: Thresholds, variance: bool):
         r2 = np.ascontiguousarray(th.r2_edges, dtype=np.float64)

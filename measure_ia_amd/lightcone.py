@@ -125,6 +125,11 @@ def rotated_ellipticity(e1, e2):
     return -e * np.cos(2 * phi_axis), -e * np.sin(2 * phi_axis)
 
 
+def _same_labels(a, b) -> bool:
+    """Whether two (position labels, shape labels, P) triples are equal."""
+    return a[2] == b[2] and np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
 def rp_cut_threshold(rp_cut) -> float:
     """Largest squared projected separation s with sqrt(s) <= rp_cut, so that
     the reference's test sqrt(s) > rp_cut (measure_m_lightcone.py:164) is s > threshold."""
@@ -189,6 +194,8 @@ class MeasureIALightcone(SimInfo):
         self.device = device
         self._th = {}
         self._memo = None   # per-measurement cache of converted data columns
+        self._share = None  # per-measurement plan of counts shared with the jackknife realisations
+        self.device_counts = 0  # device counts made so far
 
     # shared post-processing of the box front-end (measure_IA_base.py:515-668)
     _store = MeasureIABox._store
@@ -334,9 +341,27 @@ class MeasureIALightcone(SimInfo):
         return lc
 
     def _count(self, mode, masks, shapes, over_h, cosmology, print_num, rp_cut=None, patches=None):
+        share = getattr(self, "_share", None)
+        key = None
+        if share is not None and masks is None:
+            ip, is_ = id(self.data["RA"]), id(self.data["RA_shape_sample"])
+            key = (mode, bool(shapes), ip, is_, bool(over_h), id(cosmology), rp_cut)
+            if patches is None and ip in share["pos"] and is_ in share["shape"]:
+                # a measurement whose jackknife realisations come next: count
+                # once with the realisations' patch tallies; its totals are
+                # the measurement and the realisations re-use the count
+                patches = self._patch_labels(share["pos"][ip], share["shape"][is_], None)[-1]
+        if key is not None and patches is not None:
+            hit = share["done"].get(key)
+            if hit is not None and _same_labels(hit[0], patches):
+                return hit[1]
         sel = self._select(masks, shapes)
         lc = self._arrays(sel, shapes, over_h, cosmology, print_num, rp_cut, patches, masks)
-        return self._engine().count_lightcone(lc, self._thresholds(mode))
+        res = self._engine().count_lightcone(lc, self._thresholds(mode))
+        self.device_counts += 1
+        if key is not None and patches is not None:
+            share["done"][key] = (patches, res)
+        return res
 
     def _centres(self, mode):
         sep = estimators.bin_centres(self.r_bins)
@@ -439,20 +464,19 @@ class MeasureIALightcone(SimInfo):
                            data_suffix, jk_group_name, rp_cut)
 
     # -- jackknife realisations (measure_jackknife.py:59-483) ----------------------------
-    def _realisation_counts(self, patches_pos, patches_shape, masks, mode, shapes, over_h, cosmology,
-                            rp_cut, num_sample_names):
-        """One device count of the patch-labelled sample; returns (patch numbers,
-        per-realisation (S+D, SxD, DD)).  Realisation i keeps the objects with
-        patch != i and, with masks, masks["Redshift"] / ["Redshift_shape_sample"]
-        (measure_jackknife.py:84-110); it is the total minus tally[i]."""
+    def _patch_labels(self, patches_pos, patches_shape, masks, warn=False):
+        """Patch labels of one realisation count: (min, max, number of patches,
+        position and shape selections, (position labels, shape labels, P)).
+        Labels run 0..num_patches-1; a shape label outside the positions' range
+        gets the extra tally num_patches and is never removed."""
         patches_pos = np.asarray(patches_pos)
         patches_shape = np.asarray(patches_shape)
         min_patch, max_patch = int(np.min(patches_pos)), int(np.max(patches_pos))
         num_patches = max_patch - min_patch + 1
-        if np.min(patches_shape) != min_patch:
+        if warn and np.min(patches_shape) != min_patch:
             print("Warning! Minimum patch number of shape sample is not equal to minimum patch number of "
                   "position sample.")
-        if np.max(patches_shape) != max_patch:
+        if warn and np.max(patches_shape) != max_patch:
             print("Warning! Maximum patch number of shape sample is not equal to maximum patch number of "
                   "position sample.")
         mp = np.ones(len(patches_pos), dtype=bool)
@@ -461,23 +485,6 @@ class MeasureIALightcone(SimInfo):
             # the reference's (patches != i) * masks["Redshift"] (raises like it on a length mismatch)
             mp = np.asarray(mp * masks["Redshift"], dtype=bool)
             ms = np.asarray(ms * masks["Redshift_shape_sample"], dtype=bool)
-        # sum(mask) of every realisation (measure_jackknife.py:89-90): the
-        # selected objects minus those of patch i, counted in one pass
-        def per_patch(labels, sel):
-            lab = labels[sel]
-            inside = (lab >= min_patch) & (lab <= max_patch)
-            return lab.size, np.bincount((lab[inside] - min_patch).astype(np.int64), minlength=num_patches)
-
-        n_s, c_s = per_patch(patches_shape, ms)
-        n_p, c_p = per_patch(patches_pos, mp)
-        for i in range(min_patch, max_patch + 1):
-            self.num_samples[f"{i}"][num_sample_names[0]] = int(n_s - c_s[i - min_patch])
-            self.num_samples[f"{i}"][num_sample_names[1]] = int(n_p - c_p[i - min_patch])
-        keys = {"Redshift": mp, "Redshift_shape_sample": ms, "RA": mp, "RA_shape_sample": ms, "DEC": mp,
-                "DEC_shape_sample": ms, "weight": mp, "weight_shape_sample": ms}
-        if shapes:
-            keys.update({"e1": ms, "e2": ms})
-        # patch labels -> 0..num_patches-1; labels outside the position range are never removed
         lp = patches_pos[mp] - min_patch
         ls = patches_shape[ms] - min_patch
         P = num_patches
@@ -485,8 +492,36 @@ class MeasureIALightcone(SimInfo):
         if outside.any():
             ls = np.where(outside, num_patches, ls)
             P = num_patches + 1
-        res = self._count(mode, keys, shapes, over_h, cosmology, False, rp_cut,
-                          patches=(lp.astype(np.int32), ls.astype(np.int32), P))
+        return min_patch, max_patch, num_patches, mp, ms, (lp.astype(np.int32), ls.astype(np.int32), P)
+
+    def _realisation_counts(self, patches_pos, patches_shape, masks, mode, shapes, over_h, cosmology,
+                            rp_cut, num_sample_names):
+        """One device count of the patch-labelled sample; returns (patch numbers,
+        per-realisation (S+D, SxD, DD)).  Realisation i keeps the objects with
+        patch != i and, with masks, masks["Redshift"] / ["Redshift_shape_sample"]
+        (measure_jackknife.py:84-110); it is the total minus tally[i]."""
+        min_patch, max_patch, num_patches, mp, ms, patches = self._patch_labels(patches_pos, patches_shape,
+                                                                                 masks, warn=True)
+        # sum(mask) of every realisation (measure_jackknife.py:89-90): the
+        # selected objects minus those of patch i, counted in one pass
+        def per_patch(labels, sel):
+            lab = labels[sel]
+            inside = (lab >= min_patch) & (lab <= max_patch)
+            return lab.size, np.bincount((lab[inside] - min_patch).astype(np.int64), minlength=num_patches)
+
+        n_s, c_s = per_patch(np.asarray(patches_shape), ms)
+        n_p, c_p = per_patch(np.asarray(patches_pos), mp)
+        for i in range(min_patch, max_patch + 1):
+            self.num_samples[f"{i}"][num_sample_names[0]] = int(n_s - c_s[i - min_patch])
+            self.num_samples[f"{i}"][num_sample_names[1]] = int(n_p - c_p[i - min_patch])
+        keys = {"Redshift": mp, "Redshift_shape_sample": ms, "RA": mp, "RA_shape_sample": ms, "DEC": mp,
+                "DEC_shape_sample": ms, "weight": mp, "weight_shape_sample": ms}
+        if shapes:
+            keys.update({"e1": ms, "e2": ms})
+        # without masks the selection is everything: the count is the one a
+        # measurement made with the same patch labels, when there was one
+        res = self._count(mode, None if masks is None else keys, shapes, over_h, cosmology, False, rp_cut,
+                          patches=patches)
         reals = {}
         for i in range(num_patches):
             k = 1 + i
@@ -721,12 +756,26 @@ class MeasureIALightcone(SimInfo):
         out["weight_shape_sample"] = S["weight_shape_sample"]
         return out
 
+    def _share_plan(self, jk_patches):
+        """Which patch labels belong to which position / shape arrays, so that
+        each measurement count (S+D, S+R, SD, SR, RD, RR) is made once with
+        the patch tallies its jackknife realisations need.  Off when a data
+        array is also a randoms array (the labels would be ambiguous)."""
+        D, R = self.data_dir, self.randoms_data
+        pos = {id(D["RA"]): jk_patches["position"], id(R["RA"]): jk_patches["randoms_position"]}
+        shape = {id(D["RA_shape_sample"]): jk_patches["shape"],
+                 id(R["RA_shape_sample"]): jk_patches["randoms_shape"]}
+        if len(pos) < 2 or len(shape) < 2:
+            return None
+        return {"pos": pos, "shape": shape, "done": {}}
+
     def _run(self, group, IA_estimator, dataset_name, corr_type, jk_patches, num_jk, errors, masks,
              masks_randoms, cosmology, over_h, rp_cut):
         mode = "rppi" if group == "w" else "rmu"
         self._memo = {}
         data, jk_patches, num_samples = self._setup(IA_estimator, masks, masks_randoms, jk_patches, num_jk,
                                                     errors)
+        self._share = self._share_plan(jk_patches) if errors and masks is None else None
         if mode == "rppi":
             measure = lambda **k: self._measure_xi_rp_pi_lightcone_brute(**k)  # noqa: E731
             pairs = lambda **k: self._count_pairs_xi_rp_pi_lightcone_brute(**k)  # noqa: E731
@@ -771,6 +820,7 @@ class MeasureIALightcone(SimInfo):
         finally:
             self.data = data
             self._memo = None
+            self._share = None
 
     def _errors(self, group, IA_estimator, dataset_name, corr_type, jk_patches, masks, cosmology, over_h,
                 rp_cut):

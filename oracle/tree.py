@@ -17,6 +17,7 @@ baseline slightly faster than the reference.
 """
 import multiprocessing as mp
 import os
+import sys
 import time
 
 import numpy as np
@@ -124,7 +125,7 @@ def _chunk(bounds):
 
 def count_pairs(prep, r_bins, second_bins, r_min, r_max, boxsize, *, mode="rppi",
                 jk_pos=None, jk_shape=None, num_box=0, periodic=True, rp_cut=None,
-                chunk_size=1000, num_nodes=1, chunk_subset=None, abs_sums=False):
+                chunk_size=1000, num_nodes=1, chunk_subset=None, abs_sums=False, progress=False):
     """Tree counter over all shape chunks, or over ``chunk_subset`` (chunk numbers).
 
     Returns the same keys as oracle.brute.count_pairs (without 'variance',
@@ -145,7 +146,17 @@ def count_pairs(prep, r_bins, second_bins, r_min, r_max, boxsize, *, mode="rppi"
     if num_nodes > 1 and len(bounds) > 1:
         ctx = mp.get_context("fork")
         with ctx.Pool(num_nodes) as p:
-            result = p.map(_chunk, bounds)
+            if progress:
+                # chunk results in order, a line about every tenth of the chunks
+                result = []
+                step = max(1, len(bounds) // 10)
+                for r in p.imap(_chunk, bounds):
+                    result.append(r)
+                    if len(result) % step == 0:
+                        print(f"tree oracle: {len(result)}/{len(bounds)} chunks, "
+                              f"{time.perf_counter() - t0:.0f} s", file=sys.stderr, flush=True)
+            else:
+                result = p.map(_chunk, bounds)
     else:
         result = [_chunk(b) for b in bounds]
     secs = time.perf_counter() - t0

@@ -161,3 +161,31 @@ def test_plain_and_fast_backends_agree(tmp_path):
         box.measure_xi_w("All", "both", 27, temp_file_path=False)
         outs.append(_read_all(fn))
     _compare(outs[0], outs[1])
+
+
+def test_repeated_calls_reuse_the_device_sort(tmp_path):
+    """A second call on unchanged data re-uses the host prep, device copy and
+    device sort (mia_prepare_sample); a change to the data, even in place,
+    is seen and counted afresh."""
+    d = _data(seed=17)
+    fn = str(tmp_path / "out.hdf5")
+    box = MeasureIABox(d, fn, None, None, [0.1, 10.0], 10, 8, 20.0, boxsize=L, verbose=False)
+    box.measure_xi_w("a", "both", 8, temp_file_path=str(tmp_path))
+    assert box.last_timing["engine"]["reused_sort"] is False
+    box.measure_xi_multipoles("m", "both", 8, temp_file_path=str(tmp_path))
+    assert box.last_timing["engine"]["reused_sort"] is False  # another binning: its own sort
+    box.measure_xi_w("b", "both", 8, temp_file_path=str(tmp_path))
+    assert box.last_timing["engine"]["reused_sort"] is True
+    out = _read_all(fn)
+    exp_w = expected_output(d, L, "", [0.1, 10.0], 10, 8, 20.0, "w", "b", "both", 8, "tree")
+    exp_m = expected_output(d, L, "", [0.1, 10.0], 10, 8, 20.0, "m", "m", "both", 8, "tree")
+    _compare({k: v for k, v in out.items() if k in exp_w}, exp_w)
+    _compare({k: v for k, v in out.items() if k in exp_m}, exp_m)
+    # in-place change of one position: a fresh sort of the new data
+    box.data["Position"][5] = (box.data["Position"][5] + 7.0) % L
+    fn2 = str(tmp_path / "out2.hdf5")
+    box.output_file_name = fn2
+    box.measure_xi_w("c", "both", 8, temp_file_path=str(tmp_path))
+    assert box.last_timing["engine"]["reused_sort"] is False
+    exp_c = expected_output(box.data, L, "", [0.1, 10.0], 10, 8, 20.0, "w", "c", "both", 8, "tree")
+    _compare(_read_all(fn2), exp_c)

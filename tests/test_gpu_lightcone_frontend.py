@@ -2,7 +2,9 @@
 
 One small lightcone (1,200 positions, 600 shapes, 3,000 randoms, 4 RA-band
 jackknife patches) through measure_xi_w and measure_xi_multipoles with both
-estimators and every correlation type (and once with over_h); the HDF5 datasets -- counts, estimators, projections, every
+estimators and every correlation type (and once with over_h), each
+(positions, shapes) combination counted once on the device for the
+measurement and its jackknife realisations together; the HDF5 datasets -- counts, estimators, projections, every
 realisation's counts and the jackknife covariance -- are compared with
 tests/reflc.py.  Pair counts of unit-weight samples must be equal; every
 other dataset within 1e-6 of its largest value.
@@ -70,6 +72,9 @@ def test_lightcone_frontend(tmp_path, group, est, corr):
     else:
         lc.measure_xi_multipoles(est, "gp", corr, jk_patches=jk, over_h=False)
         mode, second = "rmu", lc.mu_r_bins
+    # every measurement count carries the patch tallies its realisations use:
+    # one device count per (positions, shapes) combination, not two
+    assert lc.device_counts == {"both": 4, "gg": 4, "g+": 3 if est == "galaxies" else 2}[corr]
     got = _read(fn)
     cosmo = default_cosmology()
     chi = {"D": comoving_distance(cosmo, 1 / (1 + data["Redshift"])),
