@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define MIA_ABI_VERSION 3
+#define MIA_ABI_VERSION 4
 
 enum mia_status {
     MIA_OK = 0,
@@ -145,6 +145,14 @@ typedef struct mia_tuning {
     int32_t graph;              /* mia_count_batch: 0 = capture the launch sequence in a
                                    hipGraph and replay it while it stays the same (the
                                    default); -1 = plain launches                        */
+    int32_t deterministic;      /* 1 (MIA_FAST): results identical bit for bit from run
+                                   to run: the cell sort keeps input order inside a cell
+                                   (stable radix sort), every wave takes a fixed share of
+                                   the work, and the waves' partial histograms are summed
+                                   in a fixed order with compensated FP64 sums instead of
+                                   atomics.  Plain launches (no graph); slower.  The sums
+                                   of the default path differ from run to run by FP64
+                                   rounding only.  Added in ABI version 4.              */
 } mia_tuning;
 
 /* Per-call statistics, filled when the pointer is non-NULL. */
@@ -162,7 +170,7 @@ typedef struct mia_stats {
                                MIA_PLAIN when its LDS histogram or key space is too big) */
     double ms_total;        /* the whole call on the device (events around it)          */
     int32_t graph;          /* 1: the call replayed a captured hipGraph                 */
-    int32_t reserved;
+    int32_t deterministic;  /* 1: the deterministic reduction ran (ABI version 4)        */
     int64_t item_capacity;  /* work-item slots of the workspace                         */
     int64_t wave_iterations;    /* pair-kernel loop trips: one 64-lane position batch
                                    against one shape                                    */

@@ -59,7 +59,8 @@ class MiaTuning(ctypes.Structure):
                 ("slabs_per_block", ctypes.c_int32), ("block_shapes", ctypes.c_int32),
                 ("item_positions", ctypes.c_int32), ("waves", ctypes.c_int32),
                 ("hist_copies", ctypes.c_int32), ("cell_shapes", ctypes.c_int32),
-                ("r_bins_per_pass", ctypes.c_int32), ("graph", ctypes.c_int32)]
+                ("r_bins_per_pass", ctypes.c_int32), ("graph", ctypes.c_int32),
+                ("deterministic", ctypes.c_int32)]
 
 
 class MiaStats(ctypes.Structure):
@@ -70,7 +71,7 @@ class MiaStats(ctypes.Structure):
                 ("columns_per_side", ctypes.c_int32), ("slabs_per_side", ctypes.c_int32),
                 ("slabs_per_block", ctypes.c_int32), ("block_shapes", ctypes.c_int32),
                 ("waves", ctypes.c_int32), ("method", ctypes.c_int32),
-                ("ms_total", ctypes.c_double), ("graph", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("ms_total", ctypes.c_double), ("graph", ctypes.c_int32), ("deterministic", ctypes.c_int32),
                 ("item_capacity", ctypes.c_int64), ("wave_iterations", ctypes.c_int64)]
 
     def as_dict(self):

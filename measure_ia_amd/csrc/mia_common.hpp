@@ -94,9 +94,11 @@ __device__ __forceinline__ WaveSchedule make_schedule(const int64_t *bounds, int
 
 // bounds[0..waves * kScheduleUnits + 2] of the schedule above, on stream st.
 void schedule_bounds(const int64_t *cost_prefix, int64_t n, int waves, int64_t *bounds, hipStream_t st);
-// the same with the item count read on the device
+// the same with the item count read on the device; all_static: every wave
+// takes only its static range (no tail claims), so the item -> wave map is
+// fixed by the costs
 void schedule_bounds(const int64_t *cost_prefix, const int64_t *n_dev, int waves, int64_t *bounds,
-                     hipStream_t st);
+                     hipStream_t st, bool all_static = false);
 
 constexpr int kMaxBins2 = 256;      // max n_2 (pi or mu bins)
 constexpr int kMaxBinsR = 256;      // max n_r
@@ -311,6 +313,7 @@ const unsigned long long *count_readback(const CountPlan &p);
 constexpr int kCountReadback = 7;
 int count_waves(const CountPlan &p);
 int count_bsz(const CountPlan &p);
+bool count_det(const CountPlan &p);
 int count_G(const CountPlan &p, int sample);
 int64_t count_item_cap(const CountPlan &p);
 // a key identifying the launch sequence (everything but caller pointers)

@@ -29,6 +29,8 @@
 // throughout, so every bin carries only the rounding of FP64 additions.
 #include "mia_common.hpp"
 
+#include <hipcub/hipcub.hpp>
+
 #include <cmath>
 #include <memory>
 #include <mutex>
@@ -259,6 +261,10 @@ struct PlanLaunch {
     unsigned long long *candidates;  // emit pass: sum of ns * np
     int emit;
     int local;                // 1: aggregate per workgroup in LDS (K1^2 keys fit)
+    // deterministic order: with it, the emit pass also writes every item's
+    // (key << 32 | plan thread); a stable sort by it fixes the order the
+    // cursor atomics gave the items of one key
+    unsigned long long *sort_key;
 };
 
 // Count (emit == 0) or emit the work items of every (block, stencil column)
@@ -347,6 +353,8 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan_items(PlanLaunch A) {
                 it.shifts = sh;
                 it.pad = 0;
                 A.items[slot + q] = it;
+                if (A.sort_key)
+                    A.sort_key[slot + q] = ((unsigned long long)(kb + k) << 32) | (unsigned long long)(P.t0 + tl);
                 // a wave pays for whole 64-lane position batches plus a fixed
                 // per-item overhead, whatever the item's true pair count
                 A.cost[slot + q] = (int64_t)B.ns * (((it.np + kWave - 1) / kWave) * kWave) + kItemOverhead;
@@ -428,6 +436,22 @@ __global__ void k_items_total(const int64_t *key_off, int64_t nkeys, int64_t cap
     }
 }
 
+// Deterministic order of the items: the permutation of a stable sort by
+// (key, plan thread) applied to the items and their costs.
+__global__ void k_iota(int32_t *a, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = (int32_t)i;
+}
+
+__global__ void k_gather_items(const int32_t *order, const int64_t *n_dev, const Item *items, const int64_t *cost,
+                               Item *items_out, int64_t *cost_out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *n_dev) return;
+    const int32_t o = order[i];
+    items_out[i] = items[o];
+    cost_out[i] = cost[o];
+}
+
 // Caller outputs from the accumulators: counts, DD (the pair count itself for
 // unit weights), S+D, their tallies, SxD, the variance and the region stats.
 __global__ void k_finalize(const FinSample *fs, const int64_t *fin0, int S) {
@@ -455,6 +479,109 @@ __global__ void k_finalize(const FinSample *fs, const int64_t *fin0, int S) {
         r[1] = (double)F.rcnt[F.K + k];
         r[2] = F.rsum[k];
         r[3] = F.rsum[F.K + k];
+    }
+}
+
+// Deterministic reduction, step 1: the histogram flushes of every wave of an
+// all-static schedule (bounds from schedule_bounds(all_static)), which are the
+// distinct planner keys among its items: the non-empty keys from key(i0) to
+// key(i1 - 1) over its range [i0, i1) of the key-sorted items (key_off:
+// nkeys + 1 item offsets), 0 for an empty range.  Their exclusive scan
+// numbers the slots.
+__device__ __forceinline__ int64_t key_of_item(const int64_t *key_off, int64_t nkeys, int64_t i) {
+    int64_t lo = 0, hi = nkeys - 1;  // largest k with key_off[k] <= i
+    while (lo < hi) {
+        const int64_t mid = (lo + hi + 1) >> 1;
+        if (key_off[mid] <= i)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// ne[k] = 1 when key k holds items; its exclusive scan counts the non-empty
+// keys below k (empty keys between a wave's first and last key get no flush).
+__global__ void k_key_nonempty(const int64_t *key_off, int64_t nkeys, int64_t *ne) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nkeys) ne[k] = key_off[k + 1] > key_off[k] ? 1 : 0;
+}
+
+__global__ void k_slot_runs(const int64_t *bounds, int W, const int64_t *key_off, int64_t nkeys,
+                            const int64_t *ne_pre, int64_t *runs) {
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= W) return;
+    const int64_t i0 = bounds[(int64_t)w * kScheduleUnits], i1 = bounds[(int64_t)(w + 1) * kScheduleUnits];
+    runs[w] = i1 > i0 ? ne_pre[key_of_item(key_off, nkeys, i1 - 1) + 1] - ne_pre[key_of_item(key_off, nkeys, i0)]
+                      : 0;
+}
+
+// Neumaier's compensated addition: s + c carries the sum with the rounding
+// error of every addition recovered in c.
+__device__ __forceinline__ void neumaier(double &s, double &c, double v) {
+    const double t = s + v;
+    c += fabs(s) >= fabs(v) ? (s - t) + v : (v - t) + s;
+    s = t;
+}
+
+// Deterministic reduction, step 2: caller outputs from the slots.  Output
+// element (tally t, bin b) of a sample sums, in slot order, the slots of that
+// sample whose key (R_s, R_p) includes region t - 1 (every slot for t = 0),
+// with compensated FP64 sums; region stats as k_finalize.
+__global__ void k_finalize_det(const FinSample *fs, const int64_t *fin0, int S, const int64_t *slot_total,
+                               const int64_t *slot_key, const unsigned long long *slot_np,
+                               const double *slot_sp, const double *slot_sx, const double *slot_dd,
+                               const double *slot_var, int BO) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= fin0[S]) return;
+    const int s = find_sample(fin0, S, i);
+    const FinSample &F = fs[s];
+    const int64_t j = i - fin0[s];
+    const int64_t nb = (int64_t)(1 + F.K) * F.B;
+    if (j >= nb) {
+        const int64_t k = j - nb;  // region k < K
+        if (F.out.region_stats) {
+            double *r = F.out.region_stats + 4 * k;
+            r[0] = (double)F.rcnt[k];
+            r[1] = (double)F.rcnt[F.K + k];
+            r[2] = F.rsum[k];
+            r[3] = F.rsum[F.K + k];
+        }
+        return;
+    }
+    const int t = (int)(j / F.B), b = (int)(j % F.B);
+    // the sample's slots: keys [s << 32, (s + 1) << 32) of the ascending key list
+    const int64_t n = *slot_total;
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (slot_key[mid] < ((int64_t)s << 32))
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    unsigned long long c = 0;
+    double sp = 0.0, spc = 0.0, dd = 0.0, ddc = 0.0, sx = 0.0, sxc = 0.0, vr = 0.0, vrc = 0.0;
+    for (int64_t q = lo; q < n; ++q) {
+        const int64_t key = slot_key[q];
+        if ((key >> 32) != s) break;
+        const int rs = (int)((key >> 16) & 0xffff), rp = (int)(key & 0xffff);
+        if (t > 0 && rs != t - 1 && rp != t - 1) continue;
+        const int64_t o = q * BO + b;
+        c += slot_np[o];
+        neumaier(sp, spc, slot_sp[o]);
+        if (F.weighted) neumaier(dd, ddc, slot_dd[o]);
+        if (t == 0) {
+            neumaier(sx, sxc, slot_sx[o]);
+            if (F.var_on) neumaier(vr, vrc, slot_var[o]);
+        }
+    }
+    F.out.npairs[j] = c;
+    if (F.out.dd) F.out.dd[j] = F.weighted ? dd + ddc : (double)c;
+    F.out.splus[j] = sp + spc;
+    if (t == 0) {
+        F.out.scross[b] = sx + sxc;
+        if (F.out.variance) F.out.variance[b] = F.var_on ? vr + vrc : 0.0;
     }
 }
 
@@ -486,6 +613,15 @@ struct PairArgs {
     // wall-clock span of the launch's busy waves: [0] max of ~start, [1] max of end
     unsigned long long *clock;
     unsigned long long *iterations;  // wave-iterations (a 64-lane position batch against one shape)
+    // deterministic reduction: the j-th histogram flush of wave w goes to
+    // slot slot0[w] + j (all output bins of the pass, plain stores) with its
+    // planner key; k_finalize_det sums the slots in slot order
+    int det, K1;
+    int64_t slot_cap;            // slots allocated (waves + planner keys bound every wave's flushes)
+    const int64_t *slot0;
+    int64_t *slot_key;
+    unsigned long long *slot_np;
+    double *slot_sp, *slot_sx, *slot_dd, *slot_var;
 };
 
 // Wave histogram layout (C = 2^copies_log2 interleaved copies of B bins;
@@ -551,6 +687,52 @@ __device__ void flush_hist(const PairArgs &P, char *hist, const SampleDev &S, in
             }
         }
     }
+}
+
+// A slot's key: sample << 32 | R_s << 16 | R_p (regions < 2^16), ascending in
+// the planner's key order.
+__device__ __forceinline__ int64_t slot_key_of(int sample, int rs, int rp) {
+    return ((int64_t)sample << 32) | ((int64_t)rs << 16) | (int64_t)rp;
+}
+
+// The deterministic flush: the wave's histogram (every bin of the pass, the
+// empty ones too) is stored into its slot instead of added by atomics.
+__device__ void flush_slot(const PairArgs &P, char *hist, int64_t slot, int64_t key, int lane) {
+    if (slot >= P.slot_cap) return;  // cannot happen: the slot plan counts every flush
+    uint32_t *cnt = (uint32_t *)hist;
+    double *sp = (double *)(hist + P.off_sp);
+    double *sx = (double *)(hist + P.off_sx);
+    double *dd = P.weighted ? (double *)(hist + P.off_dd) : nullptr;
+    double *vr = P.var ? (double *)(hist + P.off_var) : nullptr;
+    const int C = 1 << P.copies_log2;
+    for (int b = lane; b < P.nbins; b += kWave) {
+        uint32_t c = 0;
+        double vsp = 0.0, vsx = 0.0, vdd = 0.0, vvr = 0.0;
+        for (int k = 0; k < C; ++k) {
+            const int h = b * C + k;
+            c += cnt[h];
+            vsp += sp[h];
+            vsx += sx[h];
+            cnt[h] = 0;
+            sp[h] = 0.0;
+            sx[h] = 0.0;
+            if (dd) {
+                vdd += dd[h];
+                dd[h] = 0.0;
+            }
+            if (vr) {
+                vvr += vr[h];
+                vr[h] = 0.0;
+            }
+        }
+        const int64_t o = slot * P.nbins_out + P.bin_off + b;
+        P.slot_np[o] = c;
+        P.slot_sp[o] = vsp;
+        P.slot_sx[o] = vsx;
+        if (dd) P.slot_dd[o] = vdd;
+        if (vr) P.slot_var[o] = vvr;
+    }
+    if (lane == 0) P.slot_key[slot] = key;
 }
 
 __device__ __forceinline__ double shift_val(int cls, double L) {
@@ -845,6 +1027,7 @@ __global__ __launch_bounds__(kThreads) void k_pairs(PairArgs P) {
     int cur_key_s = -1, cur_rs = -1, cur_rp = -1;
     const SampleDev *Sp = nullptr;
     SampleView S{};
+    int64_t slot = P.det ? P.slot0[w] : 0;
     uint32_t nre = 0;
     uint64_t iterations = 0;  // wave-uniform
     do {
@@ -857,7 +1040,12 @@ __global__ __launch_bounds__(kThreads) void k_pairs(PairArgs P) {
             const int s0 = bp[0], ns = bp[1], sample = bp[4], rs = bp[5];
             if (sample != cur_key_s || rs != cur_rs || rp != cur_rp) {
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                if (Sp) flush_hist(P, hist, *Sp, cur_rs, cur_rp, lane);
+                if (Sp) {
+                    if (P.det)
+                        flush_slot(P, hist, slot++, slot_key_of(cur_key_s, cur_rs, cur_rp), lane);
+                    else
+                        flush_hist(P, hist, *Sp, cur_rs, cur_rp, lane);
+                }
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
                 if (sample != cur_key_s) {
                     Sp = &P.samples[sample];
@@ -880,7 +1068,12 @@ __global__ __launch_bounds__(kThreads) void k_pairs(PairArgs P) {
         }
     } while (sch.next(P.queue, lane, it0, it1));
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    if (Sp) flush_hist(P, hist, *Sp, cur_rs, cur_rp, lane);
+    if (Sp) {
+        if (P.det)
+            flush_slot(P, hist, slot, slot_key_of(cur_key_s, cur_rs, cur_rp), lane);
+        else
+            flush_hist(P, hist, *Sp, cur_rs, cur_rp, lane);
+    }
     if (nre) atomicAdd(P.rechecks, (unsigned long long)nre);
     if (lane == 0) {
         if (iterations) atomicAdd(P.iterations, (unsigned long long)iterations);
@@ -1109,6 +1302,8 @@ struct CountPlan {
     double margin = 0;               // largest geometric slack of the samples' grids
     PairConfig pc;
     bool any_w = false, any_var = false;
+    bool det = false;                // deterministic reduction (slots instead of atomics)
+    int64_t slot_cap = 0;            // slots: at most waves + planner keys
     std::vector<double> T, X;        // thresholds (X padded to n_2 + 1)
     std::vector<PlanSample> ps;
     std::vector<SampleDev> sd;
@@ -1133,14 +1328,24 @@ struct CountPlan {
     unsigned long long *words = nullptr;   // [kCountReadback] read back after the count
     int *queue = nullptr;
     unsigned long long *acc = nullptr;
+    int64_t *runs = nullptr, *slot0 = nullptr, *slot_key = nullptr, *ne = nullptr, *ne_pre = nullptr;
+    unsigned long long *skey = nullptr, *skey2 = nullptr;   // deterministic item order
+    int32_t *sidx = nullptr, *sidx2 = nullptr;
+    Item *items2 = nullptr;
+    int64_t *cost2 = nullptr;
+    void *stmp = nullptr;
+    size_t stmp_bytes = 0;
+    unsigned long long *slot_np = nullptr;
+    double *slot_sp = nullptr, *slot_sx = nullptr, *slot_dd = nullptr, *slot_var = nullptr;
     ScanBufs<int32_t> scan_nb;
-    ScanBufs<int64_t> scan_keys, scan_cost;
+    ScanBufs<int64_t> scan_keys, scan_cost, scan_runs, scan_ne;
     std::vector<int> G;
     std::string key;
 };
 
 int count_waves(const CountPlan &p) { return p.pc.waves; }
 int count_bsz(const CountPlan &p) { return p.bsz; }
+bool count_det(const CountPlan &p) { return p.det; }
 int count_G(const CountPlan &p, int s) { return p.G[s]; }
 int64_t count_item_cap(const CountPlan &p) { return p.item_cap; }
 std::string count_key(const CountPlan &p) { return p.key; }
@@ -1176,6 +1381,7 @@ CountPlan *plan_count(Ctx &ctx, const std::vector<SortedSample> &ss, const mia_b
     p.any_var = bins.accumulate_variance != 0;
     for (const auto &x : ss) p.any_w = p.any_w || x.weighted;
     p.pc = pair_config(bins, p.any_w, p.any_var, tuning, ctx.cu_count);
+    p.det = tuning && tuning->deterministic != 0;
 
     p.ps.resize(S);
     p.sd.resize(S);
@@ -1239,6 +1445,8 @@ CountPlan *plan_count(Ctx &ctx, const std::vector<SortedSample> &ss, const mia_b
     p.acc_total = p.acc0[S];
     MIA_REQUIRE((double)S * p.K1 * p.K1 < 2e9, "too many planner keys (samples x regions^2)");
     p.nkeys = (int64_t)S * p.K1 * p.K1;
+    p.slot_cap = p.det ? (int64_t)p.pc.waves + p.nkeys : 0;
+    MIA_REQUIRE(!p.det || p.nthreads < ((int64_t)1 << 32), "plan too large for the deterministic item order");
     const size_t plan_lds = (size_t)p.K1 * p.K1 * (sizeof(int64_t) + sizeof(int32_t));
     p.plan_local = plan_lds <= 64 * 1024 ? 1 : 0;
     // item capacity: the caller's (grown after an overflow), at least 3x the
@@ -1266,6 +1474,13 @@ CountPlan *plan_count(Ctx &ctx, const std::vector<SortedSample> &ss, const mia_b
         put(&ss[si].weighted, sizeof(int));
     }
     return P.release();
+}
+
+// bits of the deterministic item sort key: 32 of plan thread + the key index
+int det_key_bits(const CountPlan &p) {
+    int b = 1;
+    while (b < 31 && ((int64_t)1 << b) < p.nkeys + 1) ++b;
+    return 32 + b;
 }
 
 size_t count_bytes(const CountPlan &p) {
@@ -1318,6 +1533,34 @@ void count_bind(CountPlan &p, Carve &c, const std::vector<SortedSample> &ss, con
     scan_carve<int64_t>(c, p.item_cap, p.scan_cost);
     p.n_items = c.take<int64_t>(1);
     p.bounds = c.take<int64_t>((size_t)p.pc.waves * kScheduleUnits + 3);
+    if (p.det) {
+        const int64_t W = p.pc.waves, sw = p.slot_cap * B;
+        p.runs = c.take<int64_t>(W + 1);
+        p.slot0 = c.take<int64_t>(W + 1);
+        scan_carve<int64_t>(c, W, p.scan_runs);
+        p.ne = c.take<int64_t>(p.nkeys + 1);
+        p.ne_pre = c.take<int64_t>(p.nkeys + 1);
+        scan_carve<int64_t>(c, p.nkeys, p.scan_ne);
+        p.slot_key = c.take<int64_t>(p.slot_cap);
+        p.slot_np = c.take<unsigned long long>(sw);
+        p.slot_sp = c.take<double>(sw);
+        p.slot_sx = c.take<double>(sw);
+        p.slot_dd = p.any_w ? c.take<double>(sw) : nullptr;
+        p.slot_var = p.any_var ? c.take<double>(sw) : nullptr;
+        p.skey = c.take<unsigned long long>(p.item_cap);
+        p.skey2 = c.take<unsigned long long>(p.item_cap);
+        p.sidx = c.take<int32_t>(p.item_cap);
+        p.sidx2 = c.take<int32_t>(p.item_cap);
+        p.items2 = c.take<Item>(p.item_cap);
+        p.cost2 = c.take<int64_t>(p.item_cap + 1);
+        size_t bytes = 0;
+        MIA_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const unsigned long long *)nullptr,
+                                                   (unsigned long long *)nullptr, (const int32_t *)nullptr,
+                                                   (int32_t *)nullptr, (int)p.item_cap, 0, det_key_bits(p),
+                                                   (hipStream_t)0));
+        p.stmp_bytes = std::max<size_t>(bytes, 256);
+        p.stmp = c.take<char>(p.stmp_bytes);
+    }
     if (dry) return;
 
     // parameter block image
@@ -1420,14 +1663,47 @@ void count_enqueue(const CountPlan &p, hipStream_t st, const hipEvent_t *ev) {
                        p.n_items, p.words + 3);
     MIA_HIP(hipGetLastError());
     L.emit = 1;
+    const Item *items = p.items;
+    const int64_t *cost = p.cost;
+    if (p.det) {
+        fill<unsigned long long>(p.skey, ~0ull, p.item_cap, st);
+        hipLaunchKernelGGL(k_iota, dim3(nblk(p.item_cap, 256)), dim3(256), 0, st, p.sidx, p.item_cap);
+        MIA_HIP(hipGetLastError());
+        L.sort_key = p.skey;
+    }
     hipLaunchKernelGGL(k_plan_items, dim3(pgrid), dim3(kPlanThreads), plan_lds, st, L);
     MIA_HIP(hipGetLastError());
-    scan_ws<int64_t>(p.cost, p.cpre, p.item_cap, p.n_items, p.scan_cost, st);
-    schedule_bounds(p.cpre, p.n_items, p.pc.waves, p.bounds, st);
+    if (p.det) {
+        // items of one key in (plan thread, emission) order: a stable sort of
+        // the whole capacity (unused slots keep the largest key and sort last)
+        size_t bytes = p.stmp_bytes;
+        MIA_HIP(hipcub::DeviceRadixSort::SortPairs(p.stmp, bytes, (const unsigned long long *)p.skey, p.skey2,
+                                                   (const int32_t *)p.sidx, p.sidx2, (int)p.item_cap, 0,
+                                                   det_key_bits(p), st));
+        hipLaunchKernelGGL(k_gather_items, dim3(nblk(p.item_cap, 256)), dim3(256), 0, st, (const int32_t *)p.sidx2,
+                           (const int64_t *)p.n_items, (const Item *)p.items, (const int64_t *)p.cost, p.items2,
+                           p.cost2);
+        MIA_HIP(hipGetLastError());
+        items = p.items2;
+        cost = p.cost2;
+    }
+    scan_ws<int64_t>(cost, p.cpre, p.item_cap, p.n_items, p.scan_cost, st);
+    schedule_bounds(p.cpre, p.n_items, p.pc.waves, p.bounds, st, p.det);
+    if (p.det) {
+        // slot numbers of every wave's flushes (slot0[W] = slots in all)
+        hipLaunchKernelGGL(k_key_nonempty, dim3(nblk(p.nkeys, 256)), dim3(256), 0, st, (const int64_t *)p.ko,
+                           p.nkeys, p.ne);
+        MIA_HIP(hipGetLastError());
+        scan_ws<int64_t>(p.ne, p.ne_pre, p.nkeys, nullptr, p.scan_ne, st);
+        hipLaunchKernelGGL(k_slot_runs, dim3(nblk(p.pc.waves, 256)), dim3(256), 0, st, (const int64_t *)p.bounds,
+                           p.pc.waves, (const int64_t *)p.ko, p.nkeys, (const int64_t *)p.ne_pre, p.runs);
+        MIA_HIP(hipGetLastError());
+        scan_ws<int64_t>(p.runs, p.slot0, p.pc.waves, nullptr, p.scan_runs, st);
+    }
     if (ev) MIA_HIP(hipEventRecord(ev[1], st));
     // pair kernel, one launch per r-bin pass
     PairArgs A = p.pc.A;
-    A.items = p.items;
+    A.items = items;
     A.blocks = p.blocks;
     A.n_items = p.n_items;
     A.bounds = p.bounds;
@@ -1437,6 +1713,16 @@ void count_enqueue(const CountPlan &p, hipStream_t st, const hipEvent_t *ev) {
     A.rechecks = p.words;
     A.clock = p.words + 4;
     A.iterations = p.words + 6;
+    A.det = p.det ? 1 : 0;
+    A.K1 = p.K1;
+    A.slot_cap = p.slot_cap;
+    A.slot0 = p.slot0;
+    A.slot_key = p.slot_key;
+    A.slot_np = p.slot_np;
+    A.slot_sp = p.slot_sp;
+    A.slot_sx = p.slot_sx;
+    A.slot_dd = p.slot_dd;
+    A.slot_var = p.slot_var;
     const int n_r = p.pc.A.n_r, n_2 = p.pc.A.n_2;
     for (int pass = 0; pass < p.pc.npass; ++pass) {
         const int r0 = pass * p.pc.nrp, r1 = std::min(n_r, r0 + p.pc.nrp);
@@ -1454,8 +1740,15 @@ void count_enqueue(const CountPlan &p, hipStream_t st, const hipEvent_t *ev) {
             launch_pairs_mode<MIA_RMU>(A, p.pc.grid, p.pc.wpb, p.pc.lds, st);
     }
     if (ev) MIA_HIP(hipEventRecord(ev[2], st));
-    hipLaunchKernelGGL(k_finalize, dim3(nblk(p.fin_total, 256)), dim3(256), 0, st,
-                       (const FinSample *)(pd + p.off_fs), fin0, S);
+    if (p.det)
+        hipLaunchKernelGGL(k_finalize_det, dim3(nblk(p.fin_total, 256)), dim3(256), 0, st,
+                           (const FinSample *)(pd + p.off_fs), fin0, S, (const int64_t *)(p.slot0 + p.pc.waves),
+                           (const int64_t *)p.slot_key, (const unsigned long long *)p.slot_np,
+                           (const double *)p.slot_sp, (const double *)p.slot_sx, (const double *)p.slot_dd,
+                           (const double *)p.slot_var, n_r * n_2);
+    else
+        hipLaunchKernelGGL(k_finalize, dim3(nblk(p.fin_total, 256)), dim3(256), 0, st,
+                           (const FinSample *)(pd + p.off_fs), fin0, S);
     MIA_HIP(hipGetLastError());
     if (ev) MIA_HIP(hipEventRecord(ev[3], st));
 }

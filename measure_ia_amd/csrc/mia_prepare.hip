@@ -31,13 +31,14 @@ __global__ void k_fill(T *p, T v, int64_t n) {
 // [0, U]; bounds[U + 1], bounds[U + 2] = static units per wave and units per
 // tail chunk.
 __global__ void k_schedule_bounds(const int64_t *cpre, int64_t n_host, const int64_t *n_dev, int64_t U,
-                                  int waves, int64_t *bounds) {
+                                  int waves, int all_static, int64_t *bounds) {
     const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (u > U + 2) return;
     const int64_t n = n_dev ? *n_dev : n_host;
     const double C = (double)cpre[n], Ud = (double)U, t = (double)u * C;
     if (u > U) {
-        const bool fine = C >= kTailFineCost * waves, tail = C >= kTailMinCost * waves;
+        const bool fine = !all_static && C >= kTailFineCost * waves;
+        const bool tail = !all_static && C >= kTailMinCost * waves;
         if (u == U + 1)
             bounds[u] = fine ? 2 : (tail ? 4 : kScheduleUnits);
         else
@@ -338,14 +339,14 @@ void fill(T *p, T v, int64_t n, hipStream_t st) {
 void schedule_bounds(const int64_t *cost_prefix, int64_t n, int waves, int64_t *bounds, hipStream_t st) {
     const int64_t U = (int64_t)waves * kScheduleUnits;
     hipLaunchKernelGGL(k_schedule_bounds, dim3((unsigned)((U + 3 + 255) / 256)), dim3(256), 0, st, cost_prefix,
-                       n, (const int64_t *)nullptr, U, waves, bounds);
+                       n, (const int64_t *)nullptr, U, waves, 0, bounds);
     MIA_HIP(hipGetLastError());
 }
 void schedule_bounds(const int64_t *cost_prefix, const int64_t *n_dev, int waves, int64_t *bounds,
-                     hipStream_t st) {
+                     hipStream_t st, bool all_static) {
     const int64_t U = (int64_t)waves * kScheduleUnits;
     hipLaunchKernelGGL(k_schedule_bounds, dim3((unsigned)((U + 3 + 255) / 256)), dim3(256), 0, st, cost_prefix,
-                       (int64_t)0, n_dev, U, waves, bounds);
+                       (int64_t)0, n_dev, U, waves, all_static ? 1 : 0, bounds);
     MIA_HIP(hipGetLastError());
 }
 

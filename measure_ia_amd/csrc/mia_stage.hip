@@ -22,6 +22,8 @@
 // size the host bounds beforehand.
 #include "mia_common.hpp"
 
+#include <hipcub/hipcub.hpp>
+
 #include <cmath>
 #include <memory>
 #include <mutex>
@@ -296,6 +298,91 @@ __global__ __launch_bounds__(kStageThreads) void k_stage_scatter(const PrepSeg *
     }
 }
 
+// Deterministic ranks (mia_tuning.deterministic): the rank the classify
+// atomic gave depends on the order in which lanes reached the counter; a
+// stable radix sort of (cell key, object index) orders every cell by input
+// index instead, and the rank becomes the object's place in that order.
+__global__ void k_stage_keys(const int2 *kr, int64_t n, int32_t *keys, int32_t *idx) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    keys[i] = kr[i].x;
+    idx[i] = (int32_t)i;
+}
+
+__global__ void k_stage_rank(const int32_t *skeys, const int32_t *sidx, int64_t n, const int32_t *start,
+                             int2 *kr) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    kr[sidx[q]].y = (int)(q - start[skeys[q]]);
+}
+
+// Deterministic per-region shape sums: classify adds them with FP64 atomics
+// in arrival order; here each cell's shapes are summed in their sorted (input)
+// order, then each region's cells in a fixed order.  Cells lie inside one
+// region (the grid is aligned to the sub-boxes); the tail slot of every
+// column holds region-0 objects.
+__global__ void k_stage_cellsum(const PrepSeg *segs, int S, int64_t keys, const int32_t *sstart,
+                                const int32_t *sidx, double *cellsum) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= keys) return;
+    int lo = 0, hi = S - 1;  // sample of global key k: shape segments at 2s + 1
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (segs[2 * mid + 1].key0 <= k)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const PrepSeg &G = segs[2 * lo + 1];
+    double w = 0.0, q = 0.0;
+    if (G.K > 0) {
+        for (int j = sstart[k]; j < sstart[k + 1]; ++j) {
+            const int64_t i = sidx[j] - G.obj0;
+            const double wi = G.w ? G.w[i] : 1.0;
+            const double ei = G.e[i];
+            w += wi;
+            q += wi * (1 - ei * ei / 2.0);
+        }
+    }
+    cellsum[2 * k] = w;
+    cellsum[2 * k + 1] = q;
+}
+
+__global__ void k_stage_regionsum(const PrepSeg *segs, const double *cellsum) {
+    const PrepSeg &G = segs[2 * blockIdx.y + 1];
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= G.K) return;
+    const Grid &g = G.g;
+    const int jk = g.jk;
+    int j[3] = {r / (jk * jk), (r / jk) % jk, r % jk};
+    const int ja = j[g.ax_a], jb = j[g.ax_b], jl = j[g.los];
+    double w = 0.0, q = 0.0;
+    for (int ca = ja * g.ncj; ca < (ja + 1) * g.ncj; ++ca)
+        for (int cb = jb * g.ncj; cb < (jb + 1) * g.ncj; ++cb) {
+            const int64_t k0 = G.key0 + (int64_t)(ca * g.nc + cb) * (g.nz + 1);
+            for (int cl = jl * g.nzj; cl < (jl + 1) * g.nzj; ++cl) {
+                w += cellsum[2 * (k0 + cl)];
+                q += cellsum[2 * (k0 + cl) + 1];
+            }
+        }
+    if (r == 0) {
+        // the objects the strict test leaves at region 0 (faces, 0 and L)
+        for (int64_t c = 0; c < (int64_t)g.nc * g.nc; ++c) {
+            const int64_t k = G.key0 + c * (g.nz + 1) + g.nz;
+            w += cellsum[2 * k];
+            q += cellsum[2 * k + 1];
+        }
+    }
+    G.rsum[r] = w;
+    G.rsum[G.K + r] = q;
+}
+
+inline int key_bits(int64_t keys) {
+    int b = 1;
+    while (b < 31 && ((int64_t)1 << b) < keys) ++b;
+    return b;
+}
+
 inline unsigned nwg(int64_t n) { return (unsigned)((n + kStageWg - 1) / kStageWg); }
 
 // Host plan of the staged prepare of S samples.
@@ -320,6 +407,13 @@ struct PrepPlan {
     int *oor = nullptr;
     unsigned long long *zero_begin = nullptr;
     size_t zero_words = 0;
+    // deterministic ranks: keys / indices before and after a stable radix sort
+    bool det = false;
+    int32_t *dkey = nullptr, *didx = nullptr, *dkey_s = nullptr, *didx_s = nullptr;
+    double *dcell = nullptr;  // per cell key: shape weight sums (deterministic region sums)
+    int kmax = 0;
+    void *dtmp = nullptr;
+    size_t dtmp_bytes = 0;
     std::string key;
 };
 
@@ -360,9 +454,12 @@ void prep_plan(PrepPlan &p, int S, const mia_sample *samples, const mia_bins &bi
     p.keys = p.key0[S];
     p.rstat = p.rs0[S];
     p.lds_regions = kmax <= kLdsRegions ? kmax : 0;
+    p.kmax = kmax;
+    p.det = tuning && tuning->deterministic != 0;
     // the launch sequence depends on the object counts, grids and region counts
     auto put = [&](const void *v, size_t n) { p.key.append((const char *)v, n); };
     put(&S, sizeof(S));
+    put(&p.det, sizeof(p.det));
     for (const PrepSeg &G : p.segs) {
         put(&G.n, sizeof(G.n));
         put(&G.g, sizeof(G.g));
@@ -395,6 +492,20 @@ std::vector<SortedSample> prep_bind(PrepPlan &p, Carve &c, const mia_sample *sam
     p.skr = c.take<int2>(p.nshp);
     p.prec = c.take<double>(4 * p.npos);
     p.srec = c.take<double>(8 * p.nshp);
+    if (p.det) {
+        const int64_t nmax = std::max<int64_t>(std::max(p.npos, p.nshp), 1);
+        p.dkey = c.take<int32_t>(nmax);
+        p.didx = c.take<int32_t>(nmax);
+        p.dkey_s = c.take<int32_t>(nmax);
+        p.didx_s = c.take<int32_t>(nmax);
+        size_t bytes = 0;
+        MIA_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const int32_t *)nullptr, (int32_t *)nullptr,
+                                                   (const int32_t *)nullptr, (int32_t *)nullptr, (int)nmax, 0,
+                                                   key_bits(p.keys + 1), (hipStream_t)0));
+        p.dtmp_bytes = std::max<size_t>(bytes, 256);
+        p.dtmp = c.take<char>(p.dtmp_bytes);
+        p.dcell = c.take<double>(2 * (p.keys + 1));
+    }
     std::vector<SortedSample> ss(S);
     for (int s = 0; s < S; ++s) {
         SortedSample &x = ss[s];
@@ -457,6 +568,31 @@ void prep_enqueue(const PrepPlan &p, hipStream_t st) {
     }
     scan_ws<int32_t>(p.pcnt, p.pstart, p.keys, nullptr, p.scan_p, st);
     scan_ws<int32_t>(p.scnt, p.sstart, p.keys, nullptr, p.scan_s, st);
+    if (p.det) {
+        for (int kind = 0; kind < 2; ++kind) {
+            const int64_t n = kind ? p.nshp : p.npos;
+            if (n <= 0) continue;
+            int2 *kr = kind ? p.skr : p.pkr;
+            const unsigned g = (unsigned)((n + 255) / 256);
+            hipLaunchKernelGGL(k_stage_keys, dim3(g), dim3(256), 0, st, (const int2 *)kr, n, p.dkey, p.didx);
+            MIA_HIP(hipGetLastError());
+            size_t bytes = p.dtmp_bytes;
+            MIA_HIP(hipcub::DeviceRadixSort::SortPairs(p.dtmp, bytes, (const int32_t *)p.dkey, p.dkey_s,
+                                                       (const int32_t *)p.didx, p.didx_s, (int)n, 0,
+                                                       key_bits(p.keys + 1), st));
+            hipLaunchKernelGGL(k_stage_rank, dim3(g), dim3(256), 0, st, (const int32_t *)p.dkey_s,
+                               (const int32_t *)p.didx_s, n, (const int32_t *)(kind ? p.sstart : p.pstart), kr);
+            MIA_HIP(hipGetLastError());
+            if (kind == 1 && p.kmax > 0) {
+                hipLaunchKernelGGL(k_stage_cellsum, dim3((unsigned)((p.keys + 255) / 256)), dim3(256), 0, st, segs,
+                                   p.S, p.keys, (const int32_t *)p.sstart, (const int32_t *)p.didx_s, p.dcell);
+                MIA_HIP(hipGetLastError());
+                hipLaunchKernelGGL(k_stage_regionsum, dim3((unsigned)((p.kmax + 63) / 64), (unsigned)p.S), dim3(64),
+                                   0, st, segs, (const double *)p.dcell);
+                MIA_HIP(hipGetLastError());
+            }
+        }
+    }
     if (wgs > 0) {
         hipLaunchKernelGGL(k_stage_scatter, dim3((unsigned)wgs), dim3(kStageThreads), 0, st, segs, wg0, nseg);
         MIA_HIP(hipGetLastError());
@@ -615,6 +751,7 @@ void fill_stats(mia_stats *stats, const RunOut &r, const CountPlan &cp, const Gr
     stats->waves = count_waves(cp);
     stats->method = MIA_FAST;
     stats->graph = r.graph;
+    stats->deterministic = count_det(cp) ? 1 : 0;
     stats->item_capacity = count_item_cap(cp);
 }
 
@@ -671,6 +808,8 @@ void stage_release(Ctx &ctx) {
 
 void stage_count(Ctx &ctx, int n, const mia_sample *samples, const mia_bins &bins, const mia_counts *outs,
                  const mia_tuning *tuning, mia_stats *stats, hipStream_t st, StageMode mode) {
+    // the deterministic sort's radix-sort launches stay out of graphs
+    if (mode == STAGE_GRAPH && tuning && tuning->deterministic) mode = STAGE_EAGER;
     for (int attempt = 0;; ++attempt) {
         PrepPlan prep;
         prep_plan(prep, n, samples, bins, tuning);

@@ -40,7 +40,7 @@ def lib():
 def test_library_exports_every_symbol(lib):
     for name in _declared_functions():
         assert hasattr(lib, name), name
-    assert lib.mia_abi_version() == 3
+    assert lib.mia_abi_version() == 4
 
 
 @pytest.mark.skipif(shutil.which("gcc") is None, reason="gcc missing")
