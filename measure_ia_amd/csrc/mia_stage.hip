@@ -45,11 +45,13 @@ constexpr int kStageItems = 4;                        // objects per thread
 constexpr int kStageWg = kStageThreads * kStageItems; // objects per workgroup
 constexpr int kLdsRegions = 4096;                     // region stats in LDS up to this many regions
 constexpr int kGroupRounds = 8;                       // cell-key groups formed per wave and object
-// Shapes are sorted by (cell, line-of-sight eighth of the slab): a shape
+// Shapes are sorted by (cell, line-of-sight 32nd of the slab): a shape
 // block's shapes then run along the line of sight, so the pairs one lane
 // makes with consecutive shapes stay longer in one (r, pi) bin and its
-// pending-slot runs flush less often.  Cells start at fine key (cell << 3).
-constexpr int kSubBits = 3;
+// pending-slot runs flush less often (pair kernels 6-31 % faster on the
+// benchmark boxes; eighths gave most of it).  Cells start at fine key
+// (cell << sbits); sbits is kSubBits unless the fine keys would pass 2^31.
+constexpr int kSubBits = 5;
 
 // One segment: the positions (kind 0) or the shapes (kind 1) of one sample.
 struct PrepSeg {
@@ -59,7 +61,7 @@ struct PrepSeg {
     int64_t key0;             // first cell key of the sample in the concatenated key arrays
     int64_t wg0;              // first workgroup of the segment
     Grid g;
-    int kind, K, sample, pad;
+    int kind, K, sample, sbits;
     int32_t *cnt;             // key counters of the kind
     const int32_t *start;     // scanned key starts of the kind
     int2 *kr;                 // (global key, rank in cell) of every object of the kind
@@ -83,7 +85,7 @@ __device__ __forceinline__ int find_seg(const int64_t *tab, int n, int64_t v) {
 }
 
 __device__ __forceinline__ int stage_axis_cell(double x, int ji, double jk_sub, int per_jk, double width, int n,
-                                               int *sub = nullptr) {
+                                               int *sub = nullptr, int sbits = 0) {
     int k, k0;
     double q;
     if (ji >= 0) {
@@ -99,8 +101,8 @@ __device__ __forceinline__ int stage_axis_cell(double x, int ji, double jk_sub, 
         k0 = clampi((int)floor(q), 0, n - 1);
         k = k0;
     }
-    // the eighth of the cell the coordinate lies in (an ordering hint only)
-    if (sub) *sub = clampi((int)((q - (double)k0) * (1 << kSubBits)), 0, (1 << kSubBits) - 1);
+    // the 2^-sbits part of the cell the coordinate lies in (an ordering hint only)
+    if (sub) *sub = clampi((int)((q - (double)k0) * (1 << sbits)), 0, (1 << sbits) - 1);
     return k;
 }
 
@@ -169,10 +171,10 @@ __global__ __launch_bounds__(kStageThreads) void k_stage_classify(const PrepSeg 
         const int ca = stage_axis_cell(x[g.ax_a], anom ? -1 : ji[g.ax_a], g.jk_sub, g.ncj, g.cw, g.nc);
         const int cb = stage_axis_cell(x[g.ax_b], anom ? -1 : ji[g.ax_b], g.jk_sub, g.ncj, g.cw, g.nc);
         int sub = 0;
-        const int cl = stage_axis_cell(x[g.los], anom ? -1 : ji[g.los], g.jk_sub, g.nzj, g.hz, g.nz, &sub);
+        const int cl = stage_axis_cell(x[g.los], anom ? -1 : ji[g.los], g.jk_sub, g.nzj, g.hz, g.nz, &sub, G.sbits);
         const int64_t cell = G.key0 + (int64_t)(ca * g.nc + cb) * (g.nz + 1) + (anom ? g.nz : cl);
-        // shapes: fine key (cell << kSubBits) + eighth of the slab (0 in the tail slot)
-        key[it] = G.kind == 1 ? (int)((cell << kSubBits) + (anom ? 0 : sub)) : (int)cell;
+        // shapes: fine key (cell << sbits) + part of the slab (0 in the tail slot)
+        key[it] = G.kind == 1 ? (int)((cell << G.sbits) + (anom ? 0 : sub)) : (int)cell;
         // per-region sums of shapes: w and w (1 - e^2/2)
         const double ei = wq[it][1];
         wq[it][1] = wq[it][0] * (1 - ei * ei / 2.0);
@@ -392,9 +394,9 @@ __global__ void k_stage_regionsum(const PrepSeg *segs, const double *cellsum) {
 }
 
 // Cell starts of the shapes from the starts of their fine keys.
-__global__ void k_coarse_starts(const int32_t *fine, int64_t keys, int32_t *coarse) {
+__global__ void k_coarse_starts(const int32_t *fine, int64_t keys, int sbits, int32_t *coarse) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k <= keys) coarse[k] = fine[k << kSubBits];
+    if (k <= keys) coarse[k] = fine[k << sbits];
 }
 
 inline int key_bits(int64_t keys) {
@@ -419,7 +421,8 @@ struct PrepPlan {
     char *param_dev = nullptr;
     std::vector<char> param;
     int32_t *pcnt = nullptr, *scnt = nullptr, *pstart = nullptr, *sstart = nullptr;
-    int32_t *sfine = nullptr;  // shapes: starts of the fine keys (cell << kSubBits) + eighth
+    int32_t *sfine = nullptr;  // shapes: starts of the fine keys (cell << sbits) + part of the slab
+    int sbits = kSubBits;
     ScanBufs<int32_t> scan_p, scan_s;
     int2 *pkr = nullptr, *skr = nullptr;
     double *prec = nullptr, *srec = nullptr;
@@ -470,8 +473,11 @@ void prep_plan(PrepPlan &p, int S, const mia_sample *samples, const mia_bins &bi
         p.npos += m.n_pos;
         p.nshp += m.n_shape;
     }
-    MIA_REQUIRE(p.npos < (1ll << 31) && p.nshp < (1ll << 31) && (p.key0[S] << kSubBits) < (1ll << 31),
-                "samples of one call hold more than 2^31 objects or 2^28 cell keys");
+    MIA_REQUIRE(p.npos < (1ll << 31) && p.nshp < (1ll << 31) && p.key0[S] < (1ll << 31),
+                "samples of one call hold more than 2^31 objects or cell keys");
+    p.sbits = kSubBits;
+    while (p.sbits > 0 && (p.key0[S] << p.sbits) >= (1ll << 31)) --p.sbits;
+    for (PrepSeg &G : p.segs) G.sbits = p.sbits;
     p.keys = p.key0[S];
     p.rstat = p.rs0[S];
     p.lds_regions = kmax <= kLdsRegions ? kmax : 0;
@@ -499,7 +505,7 @@ std::vector<SortedSample> prep_bind(PrepPlan &p, Carve &c, const mia_sample *sam
     p.param_dev = c.take<char>(p.param_bytes);
     char *z0 = c.dry ? nullptr : c.base + c.off;
     p.pcnt = c.take<int32_t>(p.keys + 1);
-    p.scnt = c.take<int32_t>((p.keys << kSubBits) + 1);
+    p.scnt = c.take<int32_t>((p.keys << p.sbits) + 1);
     p.rcnt = c.take<unsigned long long>(p.rstat);
     p.rsum = c.take<double>(p.rstat);
     p.oor = c.take<int>(S);
@@ -507,9 +513,9 @@ std::vector<SortedSample> prep_bind(PrepPlan &p, Carve &c, const mia_sample *sam
     p.zero_words = c.dry ? 0 : (size_t)((c.base + c.off) - z0) / 8;
     p.pstart = c.take<int32_t>(p.keys + 1);
     p.sstart = c.take<int32_t>(p.keys + 1);
-    p.sfine = c.take<int32_t>((p.keys << kSubBits) + 1);
+    p.sfine = c.take<int32_t>((p.keys << p.sbits) + 1);
     scan_carve<int32_t>(c, p.keys, p.scan_p);
-    scan_carve<int32_t>(c, p.keys << kSubBits, p.scan_s);
+    scan_carve<int32_t>(c, p.keys << p.sbits, p.scan_s);
     p.pkr = c.take<int2>(p.npos);
     p.skr = c.take<int2>(p.nshp);
     p.prec = c.take<double>(4 * p.npos);
@@ -523,7 +529,7 @@ std::vector<SortedSample> prep_bind(PrepPlan &p, Carve &c, const mia_sample *sam
         size_t bytes = 0;
         MIA_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const int32_t *)nullptr, (int32_t *)nullptr,
                                                    (const int32_t *)nullptr, (int32_t *)nullptr, (int)nmax, 0,
-                                                   key_bits((p.keys << kSubBits) + 1), (hipStream_t)0));
+                                                   key_bits((p.keys << p.sbits) + 1), (hipStream_t)0));
         p.dtmp_bytes = std::max<size_t>(bytes, 256);
         p.dtmp = c.take<char>(p.dtmp_bytes);
         p.dcell = c.take<double>(2 * (p.keys + 1));
@@ -589,9 +595,9 @@ void prep_enqueue(const PrepPlan &p, hipStream_t st) {
         MIA_HIP(hipGetLastError());
     }
     scan_ws<int32_t>(p.pcnt, p.pstart, p.keys, nullptr, p.scan_p, st);
-    scan_ws<int32_t>(p.scnt, p.sfine, p.keys << kSubBits, nullptr, p.scan_s, st);
+    scan_ws<int32_t>(p.scnt, p.sfine, p.keys << p.sbits, nullptr, p.scan_s, st);
     hipLaunchKernelGGL(k_coarse_starts, dim3((unsigned)((p.keys + 1 + 255) / 256)), dim3(256), 0, st,
-                       (const int32_t *)p.sfine, p.keys, p.sstart);
+                       (const int32_t *)p.sfine, p.keys, p.sbits, p.sstart);
     MIA_HIP(hipGetLastError());
     if (p.det) {
         for (int kind = 0; kind < 2; ++kind) {
@@ -604,7 +610,7 @@ void prep_enqueue(const PrepPlan &p, hipStream_t st) {
             size_t bytes = p.dtmp_bytes;
             MIA_HIP(hipcub::DeviceRadixSort::SortPairs(p.dtmp, bytes, (const int32_t *)p.dkey, p.dkey_s,
                                                        (const int32_t *)p.didx, p.didx_s, (int)n, 0,
-                                                       key_bits((p.keys << kSubBits) + 1), st));
+                                                       key_bits((p.keys << p.sbits) + 1), st));
             hipLaunchKernelGGL(k_stage_rank, dim3(g), dim3(256), 0, st, (const int32_t *)p.dkey_s,
                                (const int32_t *)p.didx_s, n, (const int32_t *)(kind ? p.sfine : p.pstart), kr);
             MIA_HIP(hipGetLastError());
