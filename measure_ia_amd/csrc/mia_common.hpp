@@ -377,6 +377,15 @@ Grid make_grid(const mia_sample &s, const mia_bins &bins, const mia_tuning *t);
 __device__ __forceinline__ int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
 
 // Index i with the reference's strict test i*L_sub < x < (i+1)*L_sub, or -1.
+// acc = keep * acc + x (one FP64 FMA, the same rounding as fma()), written
+// into acc's own register: for fma(keep, acc, x) the compiler picks the
+// two-operand v_fmac form, which overwrites x's register, and then copies the
+// result back into acc's register on every pair.
+__device__ __forceinline__ double fma_into(double keep, double acc, double x) {
+    asm("v_fma_f64 %0, %1, %0, %2" : "+v"(acc) : "v"(keep), "v"(x));
+    return acc;
+}
+
 __device__ __forceinline__ int strict_axis_index(double x, double Lsub, int jk) {
     if (!(x > 0.0) || !(x < Lsub * jk + Lsub)) return -1;
     double q = x / Lsub;

@@ -927,10 +927,10 @@ __device__ __forceinline__ void one_pair(const PairArgs &P, const SampleView &S,
         const double keep = nw ? 0.0 : 1.0;
         pd.b = b;
         pd.c = (nw ? 0u : pd.c) + 1u;
-        pd.sp = fma(keep, pd.sp, ep);
-        pd.sx = fma(keep, pd.sx, ex);
-        if (WEIGHTED) pd.dd = fma(keep, pd.dd, dd);
-        if (VAR) pd.vr = fma(keep, pd.vr, vr);
+        pd.sp = fma_into(keep, pd.sp, ep);
+        pd.sx = fma_into(keep, pd.sx, ex);
+        if (WEIGHTED) pd.dd = fma_into(keep, pd.dd, dd);
+        if (VAR) pd.vr = fma_into(keep, pd.vr, vr);
     }
 }
 

@@ -768,10 +768,10 @@ __device__ __forceinline__ void lc_pair(const LcPairArgs &P, bool valid, double 
     const double keep = nw ? 0.0 : 1.0;
     pd.b = b;
     pd.c = (nw ? 0u : pd.c) + 1u;
-    pd.dd = fma(keep, pd.dd, pw * qw);
+    pd.dd = fma_into(keep, pd.dd, pw * qw);
     if (SHAPES) {
-        pd.sp = fma(keep, pd.sp, ep);
-        pd.sx = fma(keep, pd.sx, ex);
+        pd.sp = fma_into(keep, pd.sp, ep);
+        pd.sx = fma_into(keep, pd.sx, ex);
     }
 }
 
