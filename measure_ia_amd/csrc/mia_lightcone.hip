@@ -101,6 +101,31 @@ inline double dkey_inv(uint64_t k) {
     return x;
 }
 
+// Line-of-sight order inside a segment: chi quantised to qmax + 1 steps over
+// the sample's range, a non-decreasing function of chi.  One radix sort by
+// (segment << bits | quantised chi) orders a sample; searches compare the
+// same quantised values, so the order they rely on is exact.
+struct ChiQ {
+    double lo, scale;
+    uint64_t qmax;
+    int bits;
+};
+
+__host__ __device__ __forceinline__ uint64_t chi_q(const ChiQ &q, double x) {
+    double t = (x - q.lo) * q.scale;
+    if (!(t > 0.0)) return 0;
+    return t >= (double)q.qmax ? q.qmax : (uint64_t)t;
+}
+
+inline ChiQ make_chiq(double lo, double hi, int seg_bits) {
+    ChiQ q;
+    q.bits = std::max(1, std::min(24, 64 - seg_bits));
+    q.qmax = ((uint64_t)1 << q.bits) - 1;
+    q.lo = lo;
+    q.scale = (double)q.qmax / std::fmax(hi - lo, 1e-300);
+    return q;
+}
+
 // ---------------------------------------------------------------------------
 // 1. bounds
 
@@ -237,8 +262,8 @@ __device__ __forceinline__ int lc_cell(const LcGrid &g, double ra, double dec) {
 
 __global__ void k_lc_keys(const double *__restrict__ ra, const double *__restrict__ dec,
                           const double *__restrict__ chi, const int32_t *__restrict__ patch,
-                          int64_t n, LcGrid g, int with_slab, uint64_t *__restrict__ chikey,
-                          uint64_t *__restrict__ segkey, int32_t *__restrict__ idx) {
+                          int64_t n, LcGrid g, int with_slab, ChiQ cq, uint64_t *__restrict__ key,
+                          int32_t *__restrict__ idx) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int cell = lc_cell(g, ra[i], dec[i]);
@@ -249,15 +274,14 @@ __global__ void k_lc_keys(const double *__restrict__ ra, const double *__restric
         int s = fs < 0 ? 0 : (fs > g.nslab - 1 ? g.nslab - 1 : (int)fs);
         k = k * (uint64_t)g.nslab + (uint64_t)s;
     }
-    chikey[i] = dkey(chi[i]);
-    segkey[i] = k;
+    key[i] = (k << cq.bits) | chi_q(cq, chi[i]);
     idx[i] = (int32_t)i;
 }
 
-__global__ void k_lc_gather_key(const uint64_t *__restrict__ segkey, const int32_t *__restrict__ idx,
-                                int64_t n, uint64_t *__restrict__ out) {
+// Segment keys of the sorted combined keys.
+__global__ void k_lc_seg_of(uint64_t *__restrict__ key, int64_t n, int bits) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = segkey[idx[i]];
+    if (i < n) key[i] >>= bits;
 }
 
 // Position record, 8 doubles: {ra, dec, chi, ax, ay, w, cos_dec, 0} with
@@ -366,8 +390,16 @@ __global__ void k_lc_blocks_emit(const int32_t *__restrict__ seg_start,
         B.np = n - b * bsz < bsz ? n - b * bsz : bsz;
         B.cell = cell;
         B.patch = patch;
-        B.chi_lo = prec[8 * (int64_t)B.p0 + 2];
-        B.chi_hi = prec[8 * (int64_t)(B.p0 + B.np - 1) + 2];
+        // the positions run along chi only to the quantisation of the sort:
+        // the block's bounds are its true extremes
+        double lo = prec[8 * (int64_t)B.p0 + 2], hi = lo;
+        for (int j = 1; j < B.np; ++j) {
+            const double c = prec[8 * (int64_t)(B.p0 + j) + 2];
+            lo = c < lo ? c : lo;
+            hi = c > hi ? c : hi;
+        }
+        B.chi_lo = lo;
+        B.chi_hi = hi;
         blocks[o + b] = B;
     }
 }
@@ -438,6 +470,7 @@ struct LcPlan {
     const int32_t *sseg;      // shape segment starts, nseg + 1
     const uint64_t *sseg_key;
     const double *schi;
+    ChiQ scq;                 // the shapes' chi quantisation (their order inside a segment)
     int item_shapes;          // ranges are split at this many shapes
     int32_t *rcount;          // count pass: ranges per stencil cell
     const int64_t *roff;      // emit pass: range offsets, total + 1
@@ -445,10 +478,13 @@ struct LcPlan {
     int emit;
 };
 
-__device__ __forceinline__ int lower_chi(const double *x, int lo, int hi, double v) {
+// First index in [lo, hi) whose quantised chi is >= q (lower) or > q (upper):
+// the segment is sorted by quantised chi, so both are exact, and every shape
+// with chi in [v_lo, v_hi] lies in [lower(q(v_lo)), upper(q(v_hi))).
+__device__ __forceinline__ int lower_chi(const double *x, const ChiQ &cq, int lo, int hi, uint64_t q) {
     while (lo < hi) {
         int mid = (lo + hi) >> 1;
-        if (x[mid] < v)
+        if (chi_q(cq, x[mid]) < q)
             lo = mid + 1;
         else
             hi = mid;
@@ -456,10 +492,10 @@ __device__ __forceinline__ int lower_chi(const double *x, int lo, int hi, double
     return lo;
 }
 
-__device__ __forceinline__ int upper_chi(const double *x, int lo, int hi, double v) {
+__device__ __forceinline__ int upper_chi(const double *x, const ChiQ &cq, int lo, int hi, uint64_t q) {
     while (lo < hi) {
         int mid = (lo + hi) >> 1;
-        if (x[mid] <= v)
+        if (chi_q(cq, x[mid]) <= q)
             lo = mid + 1;
         else
             hi = mid;
@@ -499,8 +535,8 @@ __global__ void k_lc_ranges(LcPlan P) {
         double clo = B.chi_lo + g.wlo, chi_hi = B.chi_hi + g.whi;
         int64_t o = P.emit ? P.roff[t] : 0;
         for (int s = P.cfs[cell]; s < P.cfs[cell + 1]; ++s) {
-            int a = lower_chi(P.schi, P.sseg[s], P.sseg[s + 1], clo);
-            int e = upper_chi(P.schi, a, P.sseg[s + 1], chi_hi);
+            int a = lower_chi(P.schi, P.scq, P.sseg[s], P.sseg[s + 1], chi_q(P.scq, clo));
+            int e = upper_chi(P.schi, P.scq, a, P.sseg[s + 1], chi_q(P.scq, chi_hi));
             int patch = (int)(P.sseg_key[s] % (uint64_t)g.P1);
             for (int k = a; k < e; k += P.item_shapes) {
                 if (P.emit) {
@@ -958,23 +994,21 @@ struct SortedLc {
 
 // Sort one sample by (segment key, chi) and return the gather order.
 int32_t *lc_sort(Ctx &ctx, const double *ra, const double *dec, const double *chi,
-                 const int32_t *patch, int64_t n, const LcGrid &g, bool slab, uint64_t max_key,
+                 const int32_t *patch, int64_t n, const LcGrid &g, bool slab, uint64_t max_key, const ChiQ &cq,
                  const std::string &tag, hipStream_t st, uint64_t **sorted_key) {
     uint64_t *ck = ctx.arena.get<uint64_t>(tag + "/ck", n);
-    uint64_t *sk = ctx.arena.get<uint64_t>(tag + "/sk", n);
     uint64_t *k2 = ctx.arena.get<uint64_t>(tag + "/k2", n);
     int32_t *i0 = ctx.arena.get<int32_t>(tag + "/i0", n);
     int32_t *i1 = ctx.arena.get<int32_t>(tag + "/i1", n);
     hipLaunchKernelGGL(k_lc_keys, dim3(nblk_(n, 256)), dim3(256), 0, st, ra, dec, chi, patch, n, g,
-                       slab ? 1 : 0, ck, sk, i0);
+                       slab ? 1 : 0, cq, ck, i0);
     MIA_HIP(hipGetLastError());
-    // chi ascending, then (stable) by segment key
-    radix_sort<uint64_t>(ctx, ck, k2, i0, i1, n, 64, st, tag + "/s1");
-    hipLaunchKernelGGL(k_lc_gather_key, dim3(nblk_(n, 256)), dim3(256), 0, st, sk, i1, n, ck);
+    // one stable sort by (segment key, quantised chi)
+    radix_sort<uint64_t>(ctx, ck, k2, i0, i1, n, bits_for(max_key) + cq.bits, st, tag + "/s");
+    hipLaunchKernelGGL(k_lc_seg_of, dim3(nblk_(n, 256)), dim3(256), 0, st, k2, n, cq.bits);
     MIA_HIP(hipGetLastError());
-    radix_sort<uint64_t>(ctx, ck, k2, i1, i0, n, bits_for(max_key), st, tag + "/s2");
     *sorted_key = k2;
-    return i0;
+    return i1;
 }
 
 void lc_segments(Ctx &ctx, const uint64_t *key, int64_t n, const std::string &tag, hipStream_t st,
@@ -1136,8 +1170,10 @@ void count_lightcone(Ctx &ctx, const mia_lc_sample &s, const mia_bins &bins, con
     uint64_t *pkey = nullptr, *skey = nullptr;
     const uint64_t pmax_key = ((uint64_t)ncells * P1) * (uint64_t)g.nslab;
     const uint64_t smax_key = (uint64_t)ncells * P1;
-    int32_t *pord = lc_sort(ctx, s.ra, s.dec, s.chi, s.patch, np_, g, true, pmax_key, "lc/p", st, &pkey);
-    int32_t *sord = lc_sort(ctx, s.ra_s, s.dec_s, s.chi_s, s.patch_s, ns_, g, false, smax_key, "lc/s", st,
+    const ChiQ pcq = make_chiq(pchi_lo, pchi_hi, bits_for(pmax_key));
+    const ChiQ scq = make_chiq(dkey_inv(hb[1].chi_lo), dkey_inv(hb[1].chi_hi), bits_for(smax_key));
+    int32_t *pord = lc_sort(ctx, s.ra, s.dec, s.chi, s.patch, np_, g, true, pmax_key, pcq, "lc/p", st, &pkey);
+    int32_t *sord = lc_sort(ctx, s.ra_s, s.dec_s, s.chi_s, s.patch_s, ns_, g, false, smax_key, scq, "lc/s", st,
                             &skey);
     double *prec = ctx.arena.get<double>("lc/prec", 8 * np_);
     hipLaunchKernelGGL(k_lc_gather_pos, dim3(nblk_(np_, 256)), dim3(256), 0, st, pord, np_, s.ra, s.dec,
@@ -1225,6 +1261,7 @@ void count_lightcone(Ctx &ctx, const mia_lc_sample &s, const mia_bins &bins, con
     L.cfs = cfs;
     L.sseg = sseg;
     L.sseg_key = sseg_key;
+    L.scq = scq;
     L.schi = schi;
     L.item_shapes = item_shapes;
     int32_t *rc = ctx.arena.get<int32_t>("lc/rc", total + 1);
