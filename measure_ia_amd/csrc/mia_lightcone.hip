@@ -278,10 +278,14 @@ __global__ void k_lc_keys(const double *__restrict__ ra, const double *__restric
     idx[i] = (int32_t)i;
 }
 
-// Segment keys of the sorted combined keys.
-__global__ void k_lc_seg_of(uint64_t *__restrict__ key, int64_t n, int bits) {
+// Segment keys of the sorted combined keys, and (q != nullptr) the quantised
+// chi of every sorted object.
+__global__ void k_lc_seg_of(uint64_t *__restrict__ key, int64_t n, int bits, uint32_t *__restrict__ q) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) key[i] >>= bits;
+    if (i >= n) return;
+    const uint64_t k = key[i];
+    if (q) q[i] = (uint32_t)(k & (((uint64_t)1 << bits) - 1));
+    key[i] = k >> bits;
 }
 
 // Position record, 8 doubles: {ra, dec, chi, ax, ay, w, cos_dec, 0} with
@@ -469,8 +473,8 @@ struct LcPlan {
     const int32_t *cfs;       // first shape segment of every cell, ncells + 1
     const int32_t *sseg;      // shape segment starts, nseg + 1
     const uint64_t *sseg_key;
-    const double *schi;
-    ChiQ scq;                 // the shapes' chi quantisation (their order inside a segment)
+    const uint32_t *sq;       // quantised chi of every sorted shape (their order inside a segment)
+    ChiQ scq;
     int item_shapes;          // ranges are split at this many shapes
     int32_t *rcount;          // count pass: ranges per stencil cell
     const int64_t *roff;      // emit pass: range offsets, total + 1
@@ -481,10 +485,10 @@ struct LcPlan {
 // First index in [lo, hi) whose quantised chi is >= q (lower) or > q (upper):
 // the segment is sorted by quantised chi, so both are exact, and every shape
 // with chi in [v_lo, v_hi] lies in [lower(q(v_lo)), upper(q(v_hi))).
-__device__ __forceinline__ int lower_chi(const double *x, const ChiQ &cq, int lo, int hi, uint64_t q) {
+__device__ __forceinline__ int lower_chi(const uint32_t *x, int lo, int hi, uint64_t q) {
     while (lo < hi) {
         int mid = (lo + hi) >> 1;
-        if (chi_q(cq, x[mid]) < q)
+        if (x[mid] < q)
             lo = mid + 1;
         else
             hi = mid;
@@ -492,10 +496,10 @@ __device__ __forceinline__ int lower_chi(const double *x, const ChiQ &cq, int lo
     return lo;
 }
 
-__device__ __forceinline__ int upper_chi(const double *x, const ChiQ &cq, int lo, int hi, uint64_t q) {
+__device__ __forceinline__ int upper_chi(const uint32_t *x, int lo, int hi, uint64_t q) {
     while (lo < hi) {
         int mid = (lo + hi) >> 1;
-        if (chi_q(cq, x[mid]) <= q)
+        if (x[mid] <= q)
             lo = mid + 1;
         else
             hi = mid;
@@ -535,8 +539,8 @@ __global__ void k_lc_ranges(LcPlan P) {
         double clo = B.chi_lo + g.wlo, chi_hi = B.chi_hi + g.whi;
         int64_t o = P.emit ? P.roff[t] : 0;
         for (int s = P.cfs[cell]; s < P.cfs[cell + 1]; ++s) {
-            int a = lower_chi(P.schi, P.scq, P.sseg[s], P.sseg[s + 1], chi_q(P.scq, clo));
-            int e = upper_chi(P.schi, P.scq, a, P.sseg[s + 1], chi_q(P.scq, chi_hi));
+            int a = lower_chi(P.sq, P.sseg[s], P.sseg[s + 1], chi_q(P.scq, clo));
+            int e = upper_chi(P.sq, a, P.sseg[s + 1], chi_q(P.scq, chi_hi));
             int patch = (int)(P.sseg_key[s] % (uint64_t)g.P1);
             for (int k = a; k < e; k += P.item_shapes) {
                 if (P.emit) {
@@ -995,7 +999,7 @@ struct SortedLc {
 // Sort one sample by (segment key, chi) and return the gather order.
 int32_t *lc_sort(Ctx &ctx, const double *ra, const double *dec, const double *chi,
                  const int32_t *patch, int64_t n, const LcGrid &g, bool slab, uint64_t max_key, const ChiQ &cq,
-                 const std::string &tag, hipStream_t st, uint64_t **sorted_key) {
+                 const std::string &tag, hipStream_t st, uint64_t **sorted_key, uint32_t *sorted_q) {
     uint64_t *ck = ctx.arena.get<uint64_t>(tag + "/ck", n);
     uint64_t *k2 = ctx.arena.get<uint64_t>(tag + "/k2", n);
     int32_t *i0 = ctx.arena.get<int32_t>(tag + "/i0", n);
@@ -1005,7 +1009,7 @@ int32_t *lc_sort(Ctx &ctx, const double *ra, const double *dec, const double *ch
     MIA_HIP(hipGetLastError());
     // one stable sort by (segment key, quantised chi)
     radix_sort<uint64_t>(ctx, ck, k2, i0, i1, n, bits_for(max_key) + cq.bits, st, tag + "/s");
-    hipLaunchKernelGGL(k_lc_seg_of, dim3(nblk_(n, 256)), dim3(256), 0, st, k2, n, cq.bits);
+    hipLaunchKernelGGL(k_lc_seg_of, dim3(nblk_(n, 256)), dim3(256), 0, st, k2, n, cq.bits, sorted_q);
     MIA_HIP(hipGetLastError());
     *sorted_key = k2;
     return i1;
@@ -1172,9 +1176,11 @@ void count_lightcone(Ctx &ctx, const mia_lc_sample &s, const mia_bins &bins, con
     const uint64_t smax_key = (uint64_t)ncells * P1;
     const ChiQ pcq = make_chiq(pchi_lo, pchi_hi, bits_for(pmax_key));
     const ChiQ scq = make_chiq(dkey_inv(hb[1].chi_lo), dkey_inv(hb[1].chi_hi), bits_for(smax_key));
-    int32_t *pord = lc_sort(ctx, s.ra, s.dec, s.chi, s.patch, np_, g, true, pmax_key, pcq, "lc/p", st, &pkey);
+    uint32_t *sq = ctx.arena.get<uint32_t>("lc/sq", ns_);
+    int32_t *pord = lc_sort(ctx, s.ra, s.dec, s.chi, s.patch, np_, g, true, pmax_key, pcq, "lc/p", st, &pkey,
+                            nullptr);
     int32_t *sord = lc_sort(ctx, s.ra_s, s.dec_s, s.chi_s, s.patch_s, ns_, g, false, smax_key, scq, "lc/s", st,
-                            &skey);
+                            &skey, sq);
     double *prec = ctx.arena.get<double>("lc/prec", 8 * np_);
     hipLaunchKernelGGL(k_lc_gather_pos, dim3(nblk_(np_, 256)), dim3(256), 0, st, pord, np_, s.ra, s.dec,
                        s.chi, s.cos_dec, s.w, hs, prec);
@@ -1209,12 +1215,9 @@ void count_lightcone(Ctx &ctx, const mia_lc_sample &s, const mia_bins &bins, con
     hipLaunchKernelGGL(k_lc_seg_emit, dim3(nblk_(ns_, 256)), dim3(256), 0, st, skey, sf, sid, ns_, sseg,
                        sseg_key);
     MIA_HIP(hipGetLastError());
-    {
-        int32_t e[2] = {(int32_t)np_, (int32_t)ns_};
-        MIA_HIP(hipMemcpyAsync(pseg + npseg, &e[0], 4, hipMemcpyHostToDevice, st));
-        MIA_HIP(hipMemcpyAsync(sseg + nsseg, &e[1], 4, hipMemcpyHostToDevice, st));
-        MIA_HIP(hipStreamSynchronize(st));  // e[] leaves scope
-    }
+    // the end of the last segment of each sample (a fill kernel: no host buffer, no sync)
+    fill<int32_t>(pseg + npseg, (int32_t)np_, 1, st);
+    fill<int32_t>(sseg + nsseg, (int32_t)ns_, 1, st);
     int32_t *cfs = ctx.arena.get<int32_t>("lc/cfs", ncells + 1);
     hipLaunchKernelGGL(k_lc_cell_first, dim3(nblk_(ncells + 1, 256)), dim3(256), 0, st, sseg_key, nsseg,
                        ncells, P1, cfs);
@@ -1262,7 +1265,7 @@ void count_lightcone(Ctx &ctx, const mia_lc_sample &s, const mia_bins &bins, con
     L.sseg = sseg;
     L.sseg_key = sseg_key;
     L.scq = scq;
-    L.schi = schi;
+    L.sq = sq;
     L.item_shapes = item_shapes;
     int32_t *rc = ctx.arena.get<int32_t>("lc/rc", total + 1);
     int64_t *rc64 = ctx.arena.get<int64_t>("lc/rc64", total + 1);
