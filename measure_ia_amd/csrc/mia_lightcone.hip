@@ -558,6 +558,21 @@ __global__ void k_lc_ranges(LcPlan P) {
     if (!P.emit) P.rcount[t] = nout;
 }
 
+// Keys (block, patch) of the ranges, and the gather of the sorted order.
+__global__ void k_lc_range_keys(const LcRange *__restrict__ r, int64_t n, int pstride, uint64_t *__restrict__ key,
+                                int32_t *__restrict__ idx) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    key[i] = (uint64_t)r[i].blk * (uint64_t)pstride + (uint64_t)r[i].patch;
+    idx[i] = (int32_t)i;
+}
+
+__global__ void k_lc_gather_ranges(const int32_t *__restrict__ idx, int64_t n, const LcRange *__restrict__ in,
+                                   LcRange *__restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = in[idx[i]];
+}
+
 // One thread per block: consecutive ranges of one shape patch form an item of
 // at most item_shapes shapes.
 __global__ void k_lc_items(const LcRange *__restrict__ ranges, const int64_t *__restrict__ roff,
@@ -1291,6 +1306,27 @@ void count_lightcone(Ctx &ctx, const mia_lc_sample &s, const mia_bins &bins, con
     if (total > 0 && nranges > 0) {
         hipLaunchKernelGGL(k_lc_ranges, dim3(nblk_(total, 256)), dim3(256), 0, st, L);
         MIA_HIP(hipGetLastError());
+    }
+    if (P > 0 && nranges > 1) {
+        // A block's ranges come in stencil-cell order, so a stencil over a
+        // patch border alternates patches row by row and the items (one
+        // patch pair each) break at every change.  A stable sort by (block,
+        // patch) keeps each block's ranges in place and groups its patches:
+        // lightcone1M went from 6.2 items per block to about one per patch.
+        uint64_t *rk = ctx.arena.get<uint64_t>("lc/rk", nranges);
+        uint64_t *rk2 = ctx.arena.get<uint64_t>("lc/rk2", nranges);
+        int32_t *ri = ctx.arena.get<int32_t>("lc/ri", nranges);
+        int32_t *ri2 = ctx.arena.get<int32_t>("lc/ri2", nranges);
+        LcRange *ranges2 = ctx.arena.get<LcRange>("lc/ranges2", nranges);
+        hipLaunchKernelGGL(k_lc_range_keys, dim3(nblk_(nranges, 256)), dim3(256), 0, st,
+                           (const LcRange *)ranges, nranges, P1 + 1, rk, ri);
+        MIA_HIP(hipGetLastError());
+        radix_sort<uint64_t>(ctx, rk, rk2, ri, ri2, nranges, bits_for((uint64_t)nblocks * (P1 + 1)), st,
+                             "lc/rsort");
+        hipLaunchKernelGGL(k_lc_gather_ranges, dim3(nblk_(nranges, 256)), dim3(256), 0, st,
+                           (const int32_t *)ri2, nranges, (const LcRange *)ranges, ranges2);
+        MIA_HIP(hipGetLastError());
+        ranges = ranges2;
     }
 
     // ---- plan: items (count, scan, emit), sorted by patch pair
