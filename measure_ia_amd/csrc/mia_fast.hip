@@ -10,7 +10,8 @@
 //               position region R_p and one periodic shift per axis.  Items
 //               are counting-sorted by (sample, R_s, R_p).
 //   2. pairs  : a persistent grid of wavefronts takes contiguous runs of the
-//               sorted list, balanced by pair count.  Each wave reads its
+//               sorted list, balanced by estimated cost, then claims tail
+//               chunks from a counter.  Each wave reads its
 //               shape block through the scalar cache (one 64-byte record per
 //               shape, wave-uniform) and streams positions through registers,
 //               64 at a time.  Separations are the reference's own FP64
@@ -22,9 +23,12 @@
 //               FP64 when it lies within a guard band of a bin edge; the
 //               arccos NaN rule is likewise checked in FP64 for nearly
 //               parallel pairs.
-//   3. flush  : a wave adds its histogram to the global FP64 accumulators of
-//               the total and of tally[R_s] / tally[R_p] only when the
-//               (R_s, R_p) key changes, i.e. a handful of times per wave.
+//   3. flush  : a wave adds its histogram to the workspace's FP64
+//               accumulators of the total and of tally[R_s] / tally[R_p]
+//               only when the (R_s, R_p) key changes, i.e. a handful of times
+//               per wave; k_finalize then writes the caller's outputs.  In
+//               the deterministic mode the flushes go to per-wave slots that
+//               k_finalize_det sums in a fixed order instead.
 // Pair counts are integers (exact, order independent); the e+ sums are FP64
 // throughout, so every bin carries only the rounding of FP64 additions.
 #include "mia_common.hpp"
