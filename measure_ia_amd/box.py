@@ -23,6 +23,7 @@ reference, temp_file_path=None is accepted (the GPU path needs no staging).
 from __future__ import annotations
 
 import os
+import threading
 import time
 from typing import Optional
 
@@ -221,16 +222,30 @@ class MeasureIABox(SimInfo):
         th = self._bins(mode)
         jk = L_subboxes or 0
         reuse = masks is None and BACKENDS[self.backend] == "fast"
-        key = (_fingerprint(self.data), ellipticity) if reuse else None
-        cache = self._cache if (key is not None and self._cache is not None and self._cache["key"] == key) else None
+        cache, key, hashing = None, None, None
+        if reuse and self._cache is not None:
+            key = (_fingerprint(self.data), ellipticity)
+            if self._cache["key"] == key:
+                cache = self._cache
         if cache is None:
             self._cache = None  # releases the previous device copy and sorts
             p = hostprep.prepare(self.data, masks, ellipticity)
             host = dict(positions=p.positions, shapes=p.shapes, axis=p.axis, e=p.e,
                         weight=p.weight, weight_shape=p.weight_shape)
-            if key is not None:
+            if reuse:
                 host = self._engine().to_device(host)
                 cache = self._cache = {"key": key, "p": p, "arrays": host, "sorted": {}}
+                if key is None:
+                    # data seen for the first time: fingerprint it in a thread
+                    # while the device counts (the library call releases the
+                    # GIL), so a single call does not pay for the hash
+                    data = self.data
+
+                    def fingerprint(entry=cache):
+                        entry["key"] = (_fingerprint(data), ellipticity)
+
+                    hashing = threading.Thread(target=fingerprint, daemon=True)
+                    hashing.start()
         else:
             p, host = cache["p"], cache["arrays"]
         if self.verbose:
@@ -252,6 +267,8 @@ class MeasureIABox(SimInfo):
             res.stats["reused_sort"] = sorted_before
         else:
             res = eng.count(sample, th, method=BACKENDS[self.backend], variance=variance, tuning=self.tuning)
+        if hashing is not None:
+            hashing.join()
         t2 = time.perf_counter()
         inv2R = 1.0 / (2 * p.R)
         counts = {"DD": res.dd[0], "SplusD": res.splus[0] * inv2R, "ScrossD": res.scross * inv2R}

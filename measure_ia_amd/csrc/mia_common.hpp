@@ -169,6 +169,7 @@ struct Grid {
     double cw = 0, hz = 0; // nominal column width and slab height
     double margin = 0;     // geometric slack for rounding at cell edges
     int G = 0;             // slabs per shape block chosen with the grid (0: decide later)
+    int pmax = 0;          // positions per work item chosen with the grid (0: the default)
     int keys() const { return nc * nc * (nz + 1); }
 };
 
@@ -248,6 +249,8 @@ struct SortedSample {
     const double *prec = nullptr;     // 4 doubles per position {a, b, l, w}, sorted order
     const double *srec = nullptr;     // 8 doubles per shape {a, b, l, ax, ay, e*w, w, 2*e*w}
     const int32_t *pstart = nullptr, *sstart = nullptr;  // keys + 1 start offsets into the arrays above
+    const int32_t *pfine = nullptr;   // (keys << sbits) + 1 starts of the positions' fine keys
+    int sbits = 0;                    // fine key = (cell << sbits) + line-of-sight part of the slab
     const unsigned long long *rcnt = nullptr;  // [2K] per region: positions, shapes
     const double *rsum = nullptr;     // [2K] per region: sum w_s, sum w_s (1 - e^2/2)
 };

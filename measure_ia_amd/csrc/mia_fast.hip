@@ -92,6 +92,9 @@ struct PlanSample {
     int pad;
     double reach_r, reach_z;
     const int32_t *sstart, *pstart;
+    const int32_t *pfine;  // starts of the positions' fine keys (cell << sbits) + line-of-sight part
+    int sbits, pad2;
+    double x_lo, x_hi;     // (r_p, pi): the pi domain, for cutting slab runs to a block's window
     const double *srec;  // shape records (block bounding boxes)
     int64_t cell0, ncells;
     int64_t t0, nt;
@@ -165,6 +168,10 @@ __device__ __forceinline__ void axis_class(double lo, double hi, double L, doubl
         cls = 3;
 }
 
+__device__ __forceinline__ double shift_val(int cls, double L) {
+    return cls == 1 ? L : (cls == 2 ? -L : 0.0);
+}
+
 // Enumerate the position ranges of one (block, stencil column) pair: slab
 // runs within reach of the block, merged while the periodic shift class and
 // the jackknife slab stay the same, plus the column's anomalous tail.  For
@@ -230,7 +237,36 @@ __device__ __forceinline__ void plan_ranges(const PlanSample &P, const Block &B,
         }
         int jz = k / g.nzj;
         if (run_k0 >= 0 && (!inc || cz != run_cls || jz != run_jz)) {
-            range(P.pstart[base + run_k0], P.pstart[base + k], region_of(g, jx, jy, run_k0), run_cls);
+            int pb = P.pstart[base + run_k0], pe = P.pstart[base + k];
+            if (run_cls != 3 && P.sbits > 0) {
+                // the run's positions are in line-of-sight order (to 2^-sbits
+                // of a slab): keep those a shape of the block can pair with,
+                // l in (zlo + sh - x_hi, zhi + sh - x_lo] for the run's shift
+                // sh, one fine bucket of slack on each side
+                const double sh = shift_val(run_cls, g.L);
+                double wlo, whi;
+                if (P.mode == MIA_RPPI) {
+                    wlo = zlo + sh - P.x_hi - 4 * m;
+                    whi = zhi + sh - P.x_lo + 4 * m;
+                } else {
+                    const double rz2 = sqrt(fmax(rr * rr - dxy2, 0.0));
+                    wlo = zlo + sh - rz2 - 4 * m;
+                    whi = zhi + sh + rz2 + 4 * m;
+                }
+                const double hq = g.hz / (double)(1 << P.sbits);
+                const int64_t f0 = (int64_t)run_k0 << P.sbits, f1 = ((int64_t)k << P.sbits) - 1;
+                const double flo = floor(wlo / hq) - 1.0, fhi = floor(whi / hq) + 1.0;
+                const int64_t fa = flo < (double)f0 ? f0 : (flo > (double)f1 ? f1 + 1 : (int64_t)flo);
+                const int64_t fb = fhi > (double)f1 ? f1 : (fhi < (double)f0 ? f0 - 1 : (int64_t)fhi);
+                const int64_t fbase = base << P.sbits;
+                if (fb < fa) {
+                    pe = pb;
+                } else {
+                    pb = P.pfine[fbase + fa];
+                    pe = P.pfine[fbase + fb + 1];
+                }
+            }
+            range(pb, pe, region_of(g, jx, jy, run_k0), run_cls);
             run_k0 = -1;
         }
         if (inc && run_k0 < 0) {
@@ -739,9 +775,6 @@ __device__ void flush_slot(const PairArgs &P, char *hist, int64_t slot, int64_t 
     if (lane == 0) P.slot_key[slot] = key;
 }
 
-__device__ __forceinline__ double shift_val(int cls, double L) {
-    return cls == 1 ? L : (cls == 2 ? -L : 0.0);
-}
 
 __device__ __forceinline__ double r2_column_order(double a2, double b2, double l2, int los) {
     if (los == 2) return (a2 + b2) + l2;
@@ -1381,7 +1414,7 @@ CountPlan *plan_count(Ctx &ctx, const std::vector<SortedSample> &ss, const mia_b
     p.X.assign(n_2 + 1, 3.0);
     for (int i = 0; i < nx2; ++i) p.X[i] = bins.x2_edges[i];
     p.bsz = tuning && tuning->block_shapes > 0 ? std::min(tuning->block_shapes, kMaxBlockShapes) : kMaxBlockShapes;
-    const int pmax = tuning && tuning->item_positions > 0 ? tuning->item_positions : 1024;
+    const int pmax_set = tuning && tuning->item_positions > 0 ? tuning->item_positions : 0;
     p.any_var = bins.accumulate_variance != 0;
     for (const auto &x : ss) p.any_w = p.any_w || x.weighted;
     p.pc = pair_config(bins, p.any_w, p.any_var, tuning, ctx.cu_count);
@@ -1425,9 +1458,11 @@ CountPlan *plan_count(Ctx &ctx, const std::vector<SortedSample> &ss, const mia_b
         Q.G = G;
         Q.kx = (int)std::ceil((reach_r + 4 * g.margin) / g.cw) + 1;
         Q.nxw = std::min(g.nc, 2 * Q.kx + 1);
-        Q.pmax = pmax;
+        Q.pmax = pmax_set > 0 ? pmax_set : (g.pmax > 0 ? g.pmax : 1024);
         Q.reach_r = reach_r;
         Q.reach_z = reach_z;
+        Q.x_lo = bins.mode == MIA_RPPI ? bins.x2_edges[0] : -reach_r;
+        Q.x_hi = bins.mode == MIA_RPPI ? bins.x2_edges[n_2] : reach_r;
         Q.sstart = x.sstart;
         Q.pstart = x.pstart;
         Q.ncells = (int64_t)g.nc * g.nc * (Q.ng + 1);
@@ -1470,7 +1505,8 @@ CountPlan *plan_count(Ctx &ctx, const std::vector<SortedSample> &ss, const mia_b
     put(&p.item_cap, sizeof(p.item_cap));
     for (int si = 0; si < S; ++si) {
         PlanSample q = p.ps[si];
-        q.sstart = q.pstart = nullptr;
+        q.sstart = q.pstart = q.pfine = nullptr;
+        q.srec = nullptr;
         put(&q, sizeof(q));
         put(&ss[si].n_pos, sizeof(int64_t));
         put(&ss[si].n_shape, sizeof(int64_t));
@@ -1585,6 +1621,8 @@ void count_bind(CountPlan &p, Carve &c, const std::vector<SortedSample> &ss, con
         PlanSample &Q = p.ps[si];
         Q.sstart = x.sstart;
         Q.pstart = x.pstart;
+        Q.pfine = x.pfine;
+        Q.sbits = x.sbits;
         Q.srec = x.srec;
         SampleDev &D = p.sd[si];
         std::memset(&D, 0, sizeof(D));

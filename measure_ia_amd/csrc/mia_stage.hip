@@ -45,12 +45,14 @@ constexpr int kStageItems = 4;                        // objects per thread
 constexpr int kStageWg = kStageThreads * kStageItems; // objects per workgroup
 constexpr int kLdsRegions = 4096;                     // region stats in LDS up to this many regions
 constexpr int kGroupRounds = 8;                       // cell-key groups formed per wave and object
-// Shapes are sorted by (cell, line-of-sight 32nd of the slab): a shape
+// Objects are sorted by (cell, line-of-sight 32nd of the slab).  A shape
 // block's shapes then run along the line of sight, so the pairs one lane
 // makes with consecutive shapes stay longer in one (r, pi) bin and its
 // pending-slot runs flush less often (pair kernels 6-31 % faster on the
-// benchmark boxes; eighths gave most of it).  Cells start at fine key
-// (cell << sbits); sbits is kSubBits unless the fine keys would pass 2^31.
+// benchmark boxes; eighths gave most of it); a column's positions run along
+// the line of sight too, so the planner can cut a slab run to the exact
+// window of a block.  Cells start at fine key (cell << sbits); sbits is
+// kSubBits unless the fine keys would pass 2^31.
 constexpr int kSubBits = 5;
 
 // One segment: the positions (kind 0) or the shapes (kind 1) of one sample.
@@ -173,8 +175,8 @@ __global__ __launch_bounds__(kStageThreads) void k_stage_classify(const PrepSeg 
         int sub = 0;
         const int cl = stage_axis_cell(x[g.los], anom ? -1 : ji[g.los], g.jk_sub, g.nzj, g.hz, g.nz, &sub, G.sbits);
         const int64_t cell = G.key0 + (int64_t)(ca * g.nc + cb) * (g.nz + 1) + (anom ? g.nz : cl);
-        // shapes: fine key (cell << sbits) + part of the slab (0 in the tail slot)
-        key[it] = G.kind == 1 ? (int)((cell << G.sbits) + (anom ? 0 : sub)) : (int)cell;
+        // fine key (cell << sbits) + part of the slab (0 in the tail slot)
+        key[it] = (int)((cell << G.sbits) + (anom ? 0 : sub));
         // per-region sums of shapes: w and w (1 - e^2/2)
         const double ei = wq[it][1];
         wq[it][1] = wq[it][0] * (1 - ei * ei / 2.0);
@@ -421,6 +423,7 @@ struct PrepPlan {
     char *param_dev = nullptr;
     std::vector<char> param;
     int32_t *pcnt = nullptr, *scnt = nullptr, *pstart = nullptr, *sstart = nullptr;
+    int32_t *pfine = nullptr;  // positions: starts of the fine keys
     int32_t *sfine = nullptr;  // shapes: starts of the fine keys (cell << sbits) + part of the slab
     int sbits = kSubBits;
     ScanBufs<int32_t> scan_p, scan_s;
@@ -504,7 +507,7 @@ std::vector<SortedSample> prep_bind(PrepPlan &p, Carve &c, const mia_sample *sam
     p.param_bytes = pc.off;
     p.param_dev = c.take<char>(p.param_bytes);
     char *z0 = c.dry ? nullptr : c.base + c.off;
-    p.pcnt = c.take<int32_t>(p.keys + 1);
+    p.pcnt = c.take<int32_t>((p.keys << p.sbits) + 1);
     p.scnt = c.take<int32_t>((p.keys << p.sbits) + 1);
     p.rcnt = c.take<unsigned long long>(p.rstat);
     p.rsum = c.take<double>(p.rstat);
@@ -512,9 +515,10 @@ std::vector<SortedSample> prep_bind(PrepPlan &p, Carve &c, const mia_sample *sam
     p.zero_begin = (unsigned long long *)z0;
     p.zero_words = c.dry ? 0 : (size_t)((c.base + c.off) - z0) / 8;
     p.pstart = c.take<int32_t>(p.keys + 1);
+    p.pfine = c.take<int32_t>((p.keys << p.sbits) + 1);
     p.sstart = c.take<int32_t>(p.keys + 1);
     p.sfine = c.take<int32_t>((p.keys << p.sbits) + 1);
-    scan_carve<int32_t>(c, p.keys, p.scan_p);
+    scan_carve<int32_t>(c, p.keys << p.sbits, p.scan_p);
     scan_carve<int32_t>(c, p.keys << p.sbits, p.scan_s);
     p.pkr = c.take<int2>(p.npos);
     p.skr = c.take<int2>(p.nshp);
@@ -549,6 +553,8 @@ std::vector<SortedSample> prep_bind(PrepPlan &p, Carve &c, const mia_sample *sam
         x.prec = p.prec;
         x.srec = p.srec;
         x.pstart = p.pstart + p.key0[s];
+        x.pfine = p.pfine + (p.key0[s] << p.sbits);
+        x.sbits = p.sbits;
         x.sstart = p.sstart + p.key0[s];
         x.rcnt = p.rcnt + p.rs0[s];
         x.rsum = p.rsum + p.rs0[s];
@@ -559,7 +565,7 @@ std::vector<SortedSample> prep_bind(PrepPlan &p, Carve &c, const mia_sample *sam
             G.axis = kind ? m.axis : nullptr;
             G.e = kind ? m.e : nullptr;
             G.cnt = kind ? p.scnt : p.pcnt;
-            G.start = kind ? p.sfine : p.pstart;
+            G.start = kind ? p.sfine : p.pfine;
             G.kr = kind ? p.skr : p.pkr;
             G.rcnt = p.rcnt + p.rs0[s];
             G.rsum = p.rsum + p.rs0[s];
@@ -594,7 +600,10 @@ void prep_enqueue(const PrepPlan &p, hipStream_t st) {
                            p.lds_regions);
         MIA_HIP(hipGetLastError());
     }
-    scan_ws<int32_t>(p.pcnt, p.pstart, p.keys, nullptr, p.scan_p, st);
+    scan_ws<int32_t>(p.pcnt, p.pfine, p.keys << p.sbits, nullptr, p.scan_p, st);
+    hipLaunchKernelGGL(k_coarse_starts, dim3((unsigned)((p.keys + 1 + 255) / 256)), dim3(256), 0, st,
+                       (const int32_t *)p.pfine, p.keys, p.sbits, p.pstart);
+    MIA_HIP(hipGetLastError());
     scan_ws<int32_t>(p.scnt, p.sfine, p.keys << p.sbits, nullptr, p.scan_s, st);
     hipLaunchKernelGGL(k_coarse_starts, dim3((unsigned)((p.keys + 1 + 255) / 256)), dim3(256), 0, st,
                        (const int32_t *)p.sfine, p.keys, p.sbits, p.sstart);
@@ -612,7 +621,7 @@ void prep_enqueue(const PrepPlan &p, hipStream_t st) {
                                                        (const int32_t *)p.didx, p.didx_s, (int)n, 0,
                                                        key_bits((p.keys << p.sbits) + 1), st));
             hipLaunchKernelGGL(k_stage_rank, dim3(g), dim3(256), 0, st, (const int32_t *)p.dkey_s,
-                               (const int32_t *)p.didx_s, n, (const int32_t *)(kind ? p.sfine : p.pstart), kr);
+                               (const int32_t *)p.didx_s, n, (const int32_t *)(kind ? p.sfine : p.pfine), kr);
             MIA_HIP(hipGetLastError());
             if (kind == 1 && p.kmax > 0) {
                 hipLaunchKernelGGL(k_stage_cellsum, dim3((unsigned)((p.keys + 255) / 256)), dim3(256), 0, st, segs,
