@@ -291,9 +291,15 @@ struct CountCall;
 // Host plan of a staged count over n sorted samples: worst-case sizes,
 // launch shapes and the pair-kernel configuration.
 struct CountPlan;
-// item_cap: at least this many work-item slots (exactly this many with cap_exact)
+// Work-item packing of the planner: one plan thread walks kGroupColumns
+// stencil columns of a shape block and packs their position ranges into items
+// of at most kItemRanges ranges (and the item's position limit).
+constexpr int kGroupColumns = 8;
+constexpr int kItemRanges = 8;
+// item_cap, range_cap: at least this many work-item slots and range records
+// (exactly this many with cap_exact)
 struct CountPlan *plan_count(Ctx &ctx, const std::vector<SortedSample> &ss, const mia_bins &bins,
-                             const mia_tuning *tuning, int64_t item_cap, bool cap_exact);
+                             const mia_tuning *tuning, int64_t item_cap, int64_t range_cap, bool cap_exact);
 void free_plan(CountPlan *p);
 // workspace bytes of the count stages (after plan_count)
 size_t count_bytes(const CountPlan &p);
@@ -309,16 +315,18 @@ void *count_param_dev(const CountPlan &p);
 // pairs and at the end.
 void count_enqueue(const CountPlan &p, hipStream_t st, const hipEvent_t *ev);
 // Device words read back after a count: [0] rechecks, [1] candidates of the
-// planned items, [2] item overflow, [3] items, [4] pair-kernel start (~wall
-// clock), [5] end, [6] wave-iterations of the pair kernel (one 64-lane
-// position batch against one shape).
+// planned items, [2] item or range overflow, [3] items, [4] pair-kernel start
+// (~wall clock), [5] end, [6] wave-iterations of the pair kernel (one 64-lane
+// position batch against one shape), [7] range records of the multi-range
+// items, [8] position ranges of the planner.
 const unsigned long long *count_readback(const CountPlan &p);
-constexpr int kCountReadback = 7;
+constexpr int kCountReadback = 9;
 int count_waves(const CountPlan &p);
 int count_bsz(const CountPlan &p);
 bool count_det(const CountPlan &p);
 int count_G(const CountPlan &p, int sample);
 int64_t count_item_cap(const CountPlan &p);
+int64_t count_range_cap(const CountPlan &p);
 // a key identifying the launch sequence (everything but caller pointers)
 std::string count_key(const CountPlan &p);
 
@@ -344,6 +352,7 @@ struct Ctx {
     hipGraphExec_t gexec = nullptr;
     std::string gkey;               // launch-sequence key of gexec
     int64_t item_cap = 0;           // grown after an item overflow
+    int64_t range_cap = 0;          // grown after a range-record overflow
     double wall_khz = 100000.0;     // device wall-clock rate
     void *scratch = nullptr;        // the device's static workspace (allocation-free entry point)
 };

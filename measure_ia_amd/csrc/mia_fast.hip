@@ -55,15 +55,25 @@ struct Block {
     int32_t sample, rs;
 };
 
+// A work item pairs one shape block with up to kItemRanges position ranges of
+// one key and one shift class (the ranges of neighbouring stencil columns are
+// packed together, so short columns still fill the 64 lanes).  Item
+// coordinates k = 0 .. np-1 run over the ranges in order.
 struct Item {
     int32_t blk;
-    int32_t p0;
+    int32_t p0;         // nr == 1: first position; nr > 1: first of nr Range records
     int32_t np;
     int16_t rp;
     uint8_t shifts;     // 2 bits per axis (a, b, l): 0 -> 0, 1 -> +L, 2 -> -L ; bit 6: per-pair wrap
-    uint8_t pad;
+    uint8_t nr;         // position ranges
 };
 static_assert(sizeof(Item) == 16, "Item layout");
+// One range of a multi-range item: item coordinates k in [end of the previous
+// range, end) are positions base + k.
+struct Range {
+    int32_t base, end;
+};
+static_assert(sizeof(Range) == 8, "Range layout");
 static_assert(sizeof(Block) == 24, "Block layout");
 
 struct SampleDev {
@@ -80,16 +90,18 @@ struct SampleDev {
 };
 
 // Planner view of one sample (per-call parameter block).  Its (column, slab
-// group) cells occupy [cell0, cell0 + ncells) of the global cell arrays, and
-// its plan threads [t0, t0 + nt) with t0 a multiple of kPlanThreads, so a
-// workgroup never straddles two samples.  nt = worst-case blocks x nxw^2.
+// group) cells occupy [cell0, cell0 + ncells) of the global cell arrays, its
+// range threads (one per block and stencil column) [t0r, t0r + ntr) and its
+// pack threads (one per block and group of stencil columns) [t0, t0 + nt),
+// both starting at multiples of kPlanThreads, so a workgroup never straddles
+// two samples.
 struct PlanSample {
     Grid g;
     int mode, periodic;
     int ng, G;           // slab groups per column, slabs per group
     int kx, nxw;         // stencil half width / width in columns
     int pmax;            // positions per work item at most
-    int pad;
+    int ngc;             // plan threads per block: groups of kGroupColumns stencil columns
     double reach_r, reach_z;
     const int32_t *sstart, *pstart;
     const int32_t *pfine;  // starts of the positions' fine keys (cell << sbits) + line-of-sight part
@@ -97,7 +109,8 @@ struct PlanSample {
     double x_lo, x_hi;     // (r_p, pi): the pi domain, for cutting slab runs to a block's window
     const double *srec;  // shape records (block bounding boxes)
     int64_t cell0, ncells;
-    int64_t t0, nt;
+    int64_t t0, nt;      // pack threads: worst-case blocks x ngc
+    int64_t t0r, ntr;    // range threads: worst-case blocks x nxw^2
 };
 
 // Finalisation view of one sample: accumulators to caller outputs.
@@ -172,6 +185,40 @@ __device__ __forceinline__ double shift_val(int cls, double L) {
     return cls == 1 ? L : (cls == 2 ? -L : 0.0);
 }
 
+// The slabs within line-of-sight reach of a block (its shapes' extent
+// [zlo, zhi]) for a column at squared transverse distance dxy2, merged into
+// runs [k0, k) of equal periodic shift class and jackknife slab:
+// close(k0, k, class) for each run in slab order.  For (r_p, pi) the runs do
+// not depend on the column.
+template <class F>
+__device__ __forceinline__ void slab_runs(const PlanSample &P, double zlo, double zhi, double dxy2, double rr,
+                                          F &&close) {
+    const Grid &g = P.g;
+    const double m = g.margin;
+    const double rz = P.reach_z + 4 * m;
+    int run_k0 = -1, run_cls = 0, run_jz = 0;
+    for (int k = 0; k <= g.nz; ++k) {
+        bool inc = false;
+        int cz = 0;
+        if (k < g.nz) {
+            double dzmin;
+            axis_class(zlo - (k + 1) * g.hz - 2 * m, zhi - k * g.hz + 2 * m, g.L, g.half, P.periodic, dzmin, cz);
+            inc = dzmin < rz;
+            if (inc && P.mode == MIA_RMU) inc = dxy2 + dzmin * dzmin < rr * rr;
+        }
+        const int jz = k / g.nzj;
+        if (run_k0 >= 0 && (!inc || cz != run_cls || jz != run_jz)) {
+            close(run_k0, k, run_cls);
+            run_k0 = -1;
+        }
+        if (inc && run_k0 < 0) {
+            run_k0 = k;
+            run_cls = cz;
+            run_jz = jz;
+        }
+    }
+}
+
 // Enumerate the position ranges of one (block, stencil column) pair: slab
 // runs within reach of the block, merged while the periodic shift class and
 // the jackknife slab stay the same, plus the column's anomalous tail.  For
@@ -179,10 +226,11 @@ __device__ __forceinline__ double shift_val(int cls, double L) {
 // local key = R_s * K1 + R_p.  Shift byte: 2 bits per axis (a, b, l) for the
 // constant shift class, bit 6 the per-pair wrap.  Reach is tested from the
 // bounding box bb of the block's shapes, not from its cells, so a column or
-// slab that no shape of the block can reach is left out.
+// slab that no shape of the block can reach is left out.  runs: the block's
+// slab runs when they were stored (count >= 0), else they are walked here.
 template <class F>
 __device__ __forceinline__ void plan_ranges(const PlanSample &P, const Block &B, const double *bb, int so, int K1,
-                                            F &&emit) {
+                                            const int32_t *runs, F &&emit) {
     const Grid &g = P.g;
     int ox = so / P.nxw, oy = so % P.nxw;
     int ia = B.col / g.nc, ib = B.col % g.nc;
@@ -222,100 +270,289 @@ __device__ __forceinline__ void plan_ranges(const PlanSample &P, const Block &B,
                                ((cz == 3 ? 0 : cz) << 4) | (wrap ? 64 : 0));
         emit(kr + rp, p_begin, p_end, rp, sh);
     };
-    // walk slabs, merging consecutive included slabs with equal (shift class, jk slab)
-    int run_k0 = -1, run_cls = 0, run_jz = 0;
-    double rz = P.reach_z + 4 * m;
-    for (int k = 0; k <= g.nz; ++k) {
-        bool inc = false;
-        int cz = 0;
-        if (k < g.nz) {
-            double dzmin;
-            axis_class(zlo - (k + 1) * g.hz - 2 * m, zhi - k * g.hz + 2 * m, g.L, g.half, P.periodic,
-                       dzmin, cz);
-            inc = dzmin < rz;
-            if (inc && P.mode == MIA_RMU) inc = dxy2 + dzmin * dzmin < rr * rr;
-        }
-        int jz = k / g.nzj;
-        if (run_k0 >= 0 && (!inc || cz != run_cls || jz != run_jz)) {
-            int pb = P.pstart[base + run_k0], pe = P.pstart[base + k];
-            if (run_cls != 3 && P.sbits > 0) {
-                // the run's positions are in line-of-sight order (to 2^-sbits
-                // of a slab): keep those a shape of the block can pair with,
-                // l in (zlo + sh - x_hi, zhi + sh - x_lo] for the run's shift
-                // sh, one fine bucket of slack on each side
-                const double sh = shift_val(run_cls, g.L);
-                double wlo, whi;
-                if (P.mode == MIA_RPPI) {
-                    wlo = zlo + sh - P.x_hi - 4 * m;
-                    whi = zhi + sh - P.x_lo + 4 * m;
-                } else {
-                    const double rz2 = sqrt(fmax(rr * rr - dxy2, 0.0));
-                    wlo = zlo + sh - rz2 - 4 * m;
-                    whi = zhi + sh + rz2 + 4 * m;
-                }
-                const double hq = g.hz / (double)(1 << P.sbits);
-                const int64_t f0 = (int64_t)run_k0 << P.sbits, f1 = ((int64_t)k << P.sbits) - 1;
-                const double flo = floor(wlo / hq) - 1.0, fhi = floor(whi / hq) + 1.0;
-                const int64_t fa = flo < (double)f0 ? f0 : (flo > (double)f1 ? f1 + 1 : (int64_t)flo);
-                const int64_t fb = fhi > (double)f1 ? f1 : (fhi < (double)f0 ? f0 - 1 : (int64_t)fhi);
-                const int64_t fbase = base << P.sbits;
-                if (fb < fa) {
-                    pe = pb;
-                } else {
-                    pb = P.pfine[fbase + fa];
-                    pe = P.pfine[fbase + fb + 1];
-                }
+    // the positions of a run of slabs [k0, k) of one shift class
+    auto run = [&](int k0, int k, int cls) {
+        int pb = P.pstart[base + k0], pe = P.pstart[base + k];
+        if (cls != 3 && P.sbits > 0) {
+            // the run's positions are in line-of-sight order (to 2^-sbits
+            // of a slab): keep those a shape of the block can pair with,
+            // l in (zlo + sh - x_hi, zhi + sh - x_lo] for the run's shift
+            // sh, one fine bucket of slack on each side
+            const double sh = shift_val(cls, g.L);
+            double wlo, whi;
+            if (P.mode == MIA_RPPI) {
+                wlo = zlo + sh - P.x_hi - 4 * m;
+                whi = zhi + sh - P.x_lo + 4 * m;
+            } else {
+                const double rz2 = sqrt(fmax(rr * rr - dxy2, 0.0));
+                wlo = zlo + sh - rz2 - 4 * m;
+                whi = zhi + sh + rz2 + 4 * m;
             }
-            range(pb, pe, region_of(g, jx, jy, run_k0), run_cls);
-            run_k0 = -1;
+            const double hq = g.hz / (double)(1 << P.sbits);
+            const int64_t f0 = (int64_t)k0 << P.sbits, f1 = ((int64_t)k << P.sbits) - 1;
+            const double flo = floor(wlo / hq) - 1.0, fhi = floor(whi / hq) + 1.0;
+            const int64_t fa = flo < (double)f0 ? f0 : (flo > (double)f1 ? f1 + 1 : (int64_t)flo);
+            const int64_t fb = fhi > (double)f1 ? f1 : (fhi < (double)f0 ? f0 - 1 : (int64_t)fhi);
+            const int64_t fbase = base << P.sbits;
+            if (fb < fa) {
+                pe = pb;
+            } else {
+                pb = P.pfine[fbase + fa];
+                pe = P.pfine[fbase + fb + 1];
+            }
         }
-        if (inc && run_k0 < 0) {
-            run_k0 = k;
-            run_cls = cz;
-            run_jz = jz;
+        range(pb, pe, region_of(g, jx, jy, k0), cls);
+    };
+    if (runs && runs[0] >= 0) {
+        // (r_p, pi): the block's slab runs, the same for every column
+        for (int r = 0; r < runs[0]; ++r) {
+            const int v = runs[1 + r];
+            run(v & 0x7ff, (v >> 11) & 0x7ff, (v >> 22) & 3);
         }
+    } else {
+        slab_runs(P, zlo, zhi, dxy2, rr, run);
     }
     // anomalous tail of the target column: region 0, any line-of-sight value
     range(P.pstart[base + g.nz], P.pstart[base + g.nz + 1], 0, 3);
 }
 
 constexpr int kPlanThreads = 256;
+constexpr int kBlockRuns = 15;          // slab runs of a block kept for its columns ((r_p, pi))
 constexpr int kPlanGridPerCu = 16;      // workgroups per CU of the planner's persistent grid
 constexpr int64_t kItemOverhead = 256;  // per-item cost, in pair evaluations
+constexpr int kOpenItems = 4;           // items a plan thread fills at once (one per key and shift)
+
+// One position range of a (block, stencil column) pair as plan_ranges
+// enumerates it.
+struct RawRange {
+    int32_t pb, pe;   // positions [pb, pe)
+    int32_t key;      // local key R_s * K1 + R_p
+    int32_t w;        // R_p | shift byte << 16
+};
+static_assert(sizeof(RawRange) == 16, "RawRange layout");
+
+struct RangeLaunch {
+    const PlanSample *ps;
+    const int64_t *t0r;       // [S + 1] first range thread of each sample (t0r[S]: total)
+    int S, K1;
+    const int32_t *boff;
+    const Block *blocks;
+    const double *bbox;
+    const int32_t *bruns;     // slab runs of every block (k_blocks)
+    RawRange *raw;            // the ranges, one chunk per workgroup
+    unsigned long long *raw_cursor;  // ranges taken (may pass raw_cap)
+    int64_t raw_cap;
+    int32_t *roff, *rcnt;     // [range threads] first range and number of ranges of the thread
+    unsigned long long *overflow;
+};
+
+// The position ranges of every (block, stencil column) pair: one thread per
+// pair counts its ranges, the workgroup reserves one chunk of the range list,
+// and each thread stores its ranges there with their first index and count.
+__global__ __launch_bounds__(kPlanThreads) void k_plan_ranges(RangeLaunch A) {
+    __shared__ int cnt_l;
+    __shared__ int64_t base_l;
+    const int64_t nvwg = (A.t0r[A.S] + kPlanThreads - 1) / kPlanThreads;
+    for (int64_t vb = blockIdx.x; vb < nvwg; vb += gridDim.x) {
+        const int s = find_sample(A.t0r, A.S, vb * kPlanThreads);
+        const PlanSample &P = A.ps[s];
+        const int64_t tl0 = vb * kPlanThreads - P.t0r;
+        const int nx2 = P.nxw * P.nxw;
+        const int b0 = A.boff[P.cell0], b1 = A.boff[P.cell0 + P.ncells];
+        // the whole virtual workgroup lies past the sample's blocks
+        if (tl0 >= P.ntr || b0 + tl0 / nx2 >= b1) continue;
+        const int64_t tl = tl0 + threadIdx.x;
+        const int blk = b0 + (int)(tl / nx2);
+        const bool active = tl < P.ntr && blk < b1;
+        const int so = (int)(tl % nx2);
+        const double *bb = A.bbox + 6 * (int64_t)blk;
+        Block B{};
+        if (active) B = A.blocks[blk];
+        __syncthreads();  // the previous virtual workgroup is done with cnt_l, base_l
+        if (threadIdx.x == 0) cnt_l = 0;
+        __syncthreads();
+        int n = 0;
+        const int32_t *br = A.bruns + (kBlockRuns + 1) * (int64_t)blk;
+        if (active) plan_ranges(P, B, bb, so, A.K1, br, [&](int, int, int, int, uint8_t) { ++n; });
+        const int off = n ? atomicAdd(&cnt_l, n) : 0;
+        __syncthreads();
+        if (threadIdx.x == 0) base_l = cnt_l ? (int64_t)atomicAdd(A.raw_cursor, (unsigned long long)cnt_l) : 0;
+        __syncthreads();
+        int64_t o = base_l + off;
+        if (o + n > A.raw_cap) {
+            if (n) atomicOr(A.overflow, 1ull);
+            n = 0;
+            o = 0;
+        }
+        if (active) {
+            A.roff[P.t0r + tl] = (int32_t)o;
+            A.rcnt[P.t0r + tl] = n;
+        }
+        if (active && n)
+            plan_ranges(P, B, bb, so, A.K1, br, [&](int k, int pb, int pe, int rp, uint8_t sh) {
+                A.raw[o++] = RawRange{pb, pe, k, (rp & 0xffff) | ((int)sh << 16)};
+            });
+    }
+}
+
+// A finished item of a plan thread.
+struct OpenItem {
+    int key, rp, np, nr;
+    int sh;
+    int32_t p[kItemRanges];  // first position of every range
+    int32_t e[kItemRanges];  // end of every range in item coordinates
+};
+
+// Pack the ranges of the stencil columns [so0, so1) of one block (range
+// threads tr0 + so) into work items: ranges of one (key, shift) go into the
+// same item, in column order, until it holds pmax positions or kItemRanges
+// ranges (a range that does not fit is split).  Up to kOpenItems items are
+// filled at once; a range of a further (key, shift) finishes the fullest.
+// close(o) is called once per finished item, in an order fixed by the data
+// alone.  The open items live in registers: every array index is a
+// compile-time constant after unrolling.
+template <class F>
+__device__ __forceinline__ void pack_items(int pmax, const RawRange *raw, const int32_t *roff, const int32_t *rcnt,
+                                           int64_t tr0, int so0, int so1, F &&close) {
+    int okey[kOpenItems], osh[kOpenItems], orp[kOpenItems], onp[kOpenItems], onr[kOpenItems];
+    int32_t op[kOpenItems][kItemRanges], oe[kOpenItems][kItemRanges];
+#pragma unroll
+    for (int i = 0; i < kOpenItems; ++i) {
+        okey[i] = -1;
+        osh[i] = orp[i] = onp[i] = onr[i] = 0;
+#pragma unroll
+        for (int j = 0; j < kItemRanges; ++j) op[i][j] = oe[i][j] = 0;
+    }
+    auto finish = [&](int s) {
+        OpenItem o;
+#pragma unroll
+        for (int i = 0; i < kOpenItems; ++i)
+            if (i == s) {
+                o.key = okey[i];
+                o.rp = orp[i];
+                o.sh = osh[i];
+                o.np = onp[i];
+                o.nr = onr[i];
+#pragma unroll
+                for (int j = 0; j < kItemRanges; ++j) {
+                    o.p[j] = op[i][j];
+                    o.e[j] = oe[i][j];
+                }
+                onp[i] = 0;
+                onr[i] = 0;
+            }
+        close(o);
+    };
+    for (int so = so0; so < so1; ++so) {
+        const int n = rcnt[tr0 + so];
+        const int64_t o0 = roff[tr0 + so];
+        for (int q = 0; q < n; ++q) {
+            const RawRange r = raw[o0 + q];
+            const int k = r.key, sh = (r.w >> 16) & 0xff, rp = (int16_t)(r.w & 0xffff);
+            int pb = r.pb;
+            const int pe = r.pe;
+            int s = -1;
+#pragma unroll
+            for (int i = 0; i < kOpenItems; ++i)
+                if (okey[i] == k && osh[i] == sh) s = i;
+            int evict = -1;
+            if (s < 0) {
+                // a free slot, else the fullest item is finished early
+#pragma unroll
+                for (int i = kOpenItems - 1; i >= 0; --i)
+                    if (okey[i] < 0) s = i;
+                if (s < 0) {
+                    s = 0;
+                    int most = onp[0];
+#pragma unroll
+                    for (int i = 1; i < kOpenItems; ++i)
+                        if (onp[i] > most) {
+                            most = onp[i];
+                            s = i;
+                        }
+                    evict = s;
+                }
+            }
+            while (evict >= 0 || pb < pe) {
+                if (evict >= 0) {
+                    finish(evict);
+                    evict = -1;
+                }
+                bool full = false;
+                int take = 0;
+#pragma unroll
+                for (int i = 0; i < kOpenItems; ++i)
+                    if (i == s) {
+                        okey[i] = k;
+                        osh[i] = sh;
+                        orp[i] = rp;
+                        take = pe - pb < pmax - onp[i] ? pe - pb : pmax - onp[i];
+#pragma unroll
+                        for (int j = 0; j < kItemRanges; ++j)
+                            if (j == onr[i]) {
+                                op[i][j] = pb;
+                                oe[i][j] = onp[i] + take;
+                            }
+                        onp[i] += take;
+                        onr[i] += 1;
+                        full = onp[i] == pmax || onr[i] == kItemRanges;
+                    }
+                pb += take;
+                if (full) evict = s;
+                if (pb >= pe && evict >= 0) {
+                    finish(evict);
+                    evict = -1;
+                }
+            }
+        }
+    }
+    for (int i = 0; i < kOpenItems; ++i) {
+        bool open = false;
+#pragma unroll
+        for (int q = 0; q < kOpenItems; ++q)
+            if (q == i) open = okey[q] >= 0 && onr[q] > 0;
+        if (open) finish(i);
+    }
+}
 
 struct PlanLaunch {
     const PlanSample *ps;
-    const int64_t *t0;        // [S + 1] first plan thread of each sample (t0[S]: total)
+    const int64_t *t0;        // [S + 1] first pack thread of each sample (t0[S]: total)
     int S;
     int K1;                   // regions + 1 of the largest sample's key space (keys per sample K1^2)
     const int32_t *boff;      // block offset of every cell (exclusive scan of the block counts)
     const Block *blocks;
-    const double *bbox;       // 6 doubles per block: bounding box of its shapes
+    const RawRange *raw;      // the ranges of every (block, column) pair
+    const int32_t *roff, *rcnt;
     unsigned long long *key_count;   // count pass
     unsigned long long *key_cursor;  // emit pass
     const int64_t *key_off;
     Item *items;
     int64_t *cost;            // emit pass: balancing cost of every item
     int64_t cap;              // item capacity
-    unsigned long long *overflow;  // set when the items exceed the capacity
+    Range *ranges;            // emit pass: range records of the multi-range items
+    unsigned long long *range_cursor;  // emit pass: range records taken (may pass range_cap)
+    int64_t range_cap;
+    unsigned long long *overflow;  // set when the items or ranges exceed their capacity
     unsigned long long *candidates;  // emit pass: sum of ns * np
     int emit;
     int local;                // 1: aggregate per workgroup in LDS (K1^2 keys fit)
     // deterministic order: with it, the emit pass also writes every item's
-    // (key << 32 | plan thread); a stable sort by it fixes the order the
+    // (key << 32 | pack thread); a stable sort by it fixes the order the
     // cursor atomics gave the items of one key
     unsigned long long *sort_key;
 };
 
-// Count (emit == 0) or emit the work items of every (block, stencil column)
-// pair of every sample, counting-sorted by key (sample, R_s, R_p).  A
-// persistent grid walks the virtual workgroups of the samples' worst-case
-// thread ranges and skips those past a sample's actual block count (known
-// only on the device).  With `local` the key counts are aggregated per
-// workgroup in LDS, so each key sees one global atomic per workgroup instead
-// of one per range; the emit pass enumerates the ranges twice (count, then
-// place) around that reservation.  Items beyond the capacity are not written
-// and raise the overflow flag (the host then grows the capacity and recounts).
+// Count (emit == 0) or emit the work items of every (block, group of
+// kGroupColumns stencil columns) pair of every sample, counting-sorted by key
+// (sample, R_s, R_p).  A persistent grid walks the virtual workgroups of the
+// samples' worst-case thread ranges and skips those past a sample's actual
+// block count (known only on the device).  With `local` the key counts are
+// aggregated per workgroup in LDS, so each key sees one global atomic per
+// workgroup instead of one per item.  The emit pass packs the items twice:
+// first to reserve the workgroup's slots (keys, range records), then to place
+// them.  Items or ranges beyond their capacity are not written and raise the
+// overflow flag (the host then grows the capacity and counts again).
 __global__ __launch_bounds__(kPlanThreads) void k_plan_items(PlanLaunch A) {
     // local: reserved bases [nk] (64-bit), then counts [nk]
     extern __shared__ int64_t lmem[];
@@ -323,6 +560,8 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan_items(PlanLaunch A) {
     int64_t *lbase = lmem;
     int32_t *lcnt = (int32_t *)(lmem + nk);
     __shared__ unsigned long long cand_l;
+    __shared__ int rcount_l;
+    __shared__ int64_t rbase_l;
     const int64_t nvwg = (A.t0[A.S] + kPlanThreads - 1) / kPlanThreads;
     const bool local = A.local != 0;
     unsigned long long cand = 0;
@@ -331,78 +570,104 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan_items(PlanLaunch A) {
         const PlanSample &P = A.ps[s];
         const int64_t tl0 = vb * kPlanThreads - P.t0;
         const int nx2 = P.nxw * P.nxw;
+        const int ngc = P.ngc;
         const int b0 = A.boff[P.cell0], b1 = A.boff[P.cell0 + P.ncells];
         // the whole virtual workgroup lies past the sample's blocks
-        if (tl0 >= P.nt || b0 + tl0 / nx2 >= b1) continue;
+        if (tl0 >= P.nt || b0 + tl0 / ngc >= b1) continue;
         const int64_t tl = tl0 + threadIdx.x;
-        const int blk = b0 + (int)(tl / nx2);
+        const int blk = b0 + (int)(tl / ngc);
         const bool active = tl < P.nt && blk < b1;
-        const int so = (int)(tl % nx2);
+        const int so0 = (int)(tl % ngc) * kGroupColumns;
+        const int so1 = so0 + kGroupColumns < nx2 ? so0 + kGroupColumns : nx2;
+        const int64_t tr0 = P.t0r + (int64_t)(blk - b0) * nx2;  // the block's first range thread
         const int kb = s * nk;
-        const int pmax = P.pmax;
-        Block B{};
-        if (active) B = A.blocks[blk];
-        if (local) {
-            __syncthreads();  // the previous virtual workgroup is done with the LDS tables
+        const int ns = active ? A.blocks[blk].ns : 0;
+        __syncthreads();  // the previous virtual workgroup is done with the LDS tables
+        if (local)
             for (int j = threadIdx.x; j < nk; j += blockDim.x) {
                 lcnt[j] = 0;
                 lbase[j] = 0;
             }
-            __syncthreads();
-        }
-        if (!A.emit || local) {
-            if (active)
-                plan_ranges(P, B, A.bbox + 6 * (int64_t)blk, so, A.K1, [&](int k, int pb, int pe, int, uint8_t) {
-                    int n = (pe - pb + pmax - 1) / pmax;
+        if (threadIdx.x == 0) rcount_l = 0;
+        __syncthreads();
+        // pass 1: key counts (count pass; emit pass with local) and the
+        // thread's range records (emit pass)
+        int nrng = 0;
+        if (active)
+            pack_items(P.pmax, A.raw, A.roff, A.rcnt, tr0, so0, so1, [&](const OpenItem &o) {
+                if (!A.emit || local) {
                     if (local)
-                        atomicAdd(&lcnt[k], n);
+                        atomicAdd(&lcnt[o.key], 1);
                     else
-                        atomicAdd(&A.key_count[kb + k], (unsigned long long)n);
-                });
-        }
-        if (local) {
-            __syncthreads();
-            for (int j = threadIdx.x; j < nk; j += blockDim.x) {
-                int c = lcnt[j];
-                if (!A.emit) {
-                    if (c) atomicAdd(&A.key_count[kb + j], (unsigned long long)c);
-                } else {
-                    lbase[j] = c ? (int64_t)atomicAdd(&A.key_cursor[kb + j], (unsigned long long)c) : 0;
-                    lcnt[j] = 0;
+                        atomicAdd(&A.key_count[kb + o.key], 1ull);
                 }
+                if (o.nr > 1) nrng += o.nr;
+            });
+        if (!A.emit) {
+            if (local) {
+                __syncthreads();
+                for (int j = threadIdx.x; j < nk; j += blockDim.x)
+                    if (lcnt[j]) atomicAdd(&A.key_count[kb + j], (unsigned long long)lcnt[j]);
             }
-            __syncthreads();
+            continue;
         }
-        if (!A.emit) continue;
-        if (active) plan_ranges(P, B, A.bbox + 6 * (int64_t)blk, so, A.K1, [&](int k, int pb, int pe, int rp, uint8_t sh) {
-            int n = (pe - pb + pmax - 1) / pmax;
-            int64_t slot = A.key_off[kb + k] +
-                           (local ? lbase[k] + (int64_t)atomicAdd(&lcnt[k], n)
-                                  : (int64_t)atomicAdd(&A.key_cursor[kb + k], (unsigned long long)n));
-            if (slot + n > A.cap) {
-                atomicOr(A.overflow, 1ull);
-                return;
+        const int roff = nrng ? atomicAdd(&rcount_l, nrng) : 0;
+        __syncthreads();
+        if (local)
+            for (int j = threadIdx.x; j < nk; j += blockDim.x) {
+                const int c = lcnt[j];
+                lbase[j] = c ? (int64_t)atomicAdd(&A.key_cursor[kb + j], (unsigned long long)c) : 0;
+                lcnt[j] = 0;
             }
-            for (int q = 0; q < n; ++q) {
-                int p = pb + q * pmax;
+        if (threadIdx.x == 0)
+            rbase_l = rcount_l ? (int64_t)atomicAdd(A.range_cursor, (unsigned long long)rcount_l) : 0;
+        __syncthreads();
+        int64_t rnext = rbase_l + roff;
+        if (rnext + nrng > A.range_cap) {
+            atomicOr(A.overflow, 1ull);
+            continue;
+        }
+        // pass 2: place
+        if (active)
+            pack_items(P.pmax, A.raw, A.roff, A.rcnt, tr0, so0, so1, [&](const OpenItem &o) {
+                const int k = o.key;
+                const int64_t slot = A.key_off[kb + k] + (local ? lbase[k] + (int64_t)atomicAdd(&lcnt[k], 1)
+                                                                : (int64_t)atomicAdd(&A.key_cursor[kb + k], 1ull));
+                if (slot >= A.cap) {
+                    atomicOr(A.overflow, 1ull);
+                    return;
+                }
                 Item it;
                 it.blk = blk;
-                it.p0 = p;
-                it.np = pe - p < pmax ? pe - p : pmax;
-                it.rp = (int16_t)rp;
-                it.shifts = sh;
-                it.pad = 0;
-                A.items[slot + q] = it;
+                it.np = o.np;
+                it.rp = (int16_t)o.rp;
+                it.shifts = (uint8_t)o.sh;
+                it.nr = (uint8_t)o.nr;
+                if (o.nr == 1) {
+                    it.p0 = o.p[0];
+                } else {
+                    it.p0 = (int32_t)rnext;
+#pragma unroll
+                    for (int i = 0; i < kItemRanges; ++i)
+                        if (i < o.nr) {
+                            Range r;
+                            r.base = o.p[i] - (i ? o.e[i - 1] : 0);
+                            r.end = o.e[i];
+                            A.ranges[rnext + i] = r;
+                        }
+                    rnext += o.nr;
+                }
+                A.items[slot] = it;
                 if (A.sort_key)
-                    A.sort_key[slot + q] = ((unsigned long long)(kb + k) << 32) | (unsigned long long)(P.t0 + tl);
+                    A.sort_key[slot] = ((unsigned long long)(kb + k) << 32) | (unsigned long long)(P.t0 + tl);
                 // a wave pays for whole 64-lane position batches plus a fixed
                 // per-item overhead, whatever the item's true pair count
-                A.cost[slot + q] = (int64_t)B.ns * (((it.np + kWave - 1) / kWave) * kWave) + kItemOverhead;
-                cand += (unsigned long long)B.ns * it.np;
-            }
-        });
+                A.cost[slot] = (int64_t)ns * (((it.np + kWave - 1) / kWave) * kWave) + kItemOverhead;
+                cand += (unsigned long long)ns * it.np;
+            });
     }
     if (!A.emit) return;
+    __syncthreads();
     if (threadIdx.x == 0) cand_l = 0;
     __syncthreads();
     if (cand) atomicAdd(&cand_l, cand);
@@ -412,9 +677,12 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan_items(PlanLaunch A) {
 
 // Shape blocks: per (column, slab group) cell of every sample, the number of
 // blocks of at most bsz shapes (count), then the blocks themselves at the
-// scanned offsets (emit).  cell0[S + 1] locates each sample's cells.
+// scanned offsets (emit) with their bounding boxes and, for (r_p, pi), their
+// slab runs (kBlockRuns + 1 ints per block: the count, -1 if they do not fit,
+// then k0 | k1 << 11 | class << 22 per run).  cell0[S + 1] locates each
+// sample's cells.
 __global__ void k_blocks(const PlanSample *ps, const int64_t *cell0, int S, int bsz,
-                         int32_t *nb, const int32_t *boff, Block *blocks, double *bbox) {
+                         int32_t *nb, const int32_t *boff, Block *blocks, double *bbox, int32_t *bruns) {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= cell0[S]) return;
     const int s = find_sample(cell0, S, c);
@@ -463,15 +731,26 @@ __global__ void k_blocks(const PlanSample *ps, const int64_t *cell0, int S, int 
             bb[2 * k] = lo[k];
             bb[2 * k + 1] = hi[k];
         }
+        if (bruns) {
+            int32_t *br = bruns + (kBlockRuns + 1) * (int64_t)(o + b);
+            int nr = 0;
+            if (P.mode == MIA_RPPI)
+                slab_runs(P, lo[2], hi[2], 0.0, 0.0, [&](int k0, int k1, int cls) {
+                    if (nr < kBlockRuns) br[1 + nr] = k0 | (k1 << 11) | (cls << 22);
+                    ++nr;
+                });
+            br[0] = P.mode == MIA_RPPI && nr <= kBlockRuns ? nr : -1;
+        }
     }
 }
 
-// n_items = min(total of the key counts, capacity); raw total kept for the host.
+// n_items = total of the key counts (0 past the capacity); raw total kept for the host.
 __global__ void k_items_total(const int64_t *key_off, int64_t nkeys, int64_t cap, int64_t *n_items,
                               unsigned long long *raw) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         const int64_t t = key_off[nkeys];
-        *n_items = t < cap ? t : cap;
+        // beyond the capacity the host counts again; nothing is scheduled
+        *n_items = t <= cap ? t : 0;
         *raw = (unsigned long long)t;
     }
 }
@@ -630,6 +909,8 @@ __global__ void k_finalize_det(const FinSample *fs, const int64_t *fin0, int S, 
 
 struct PairArgs {
     const Item *items;
+    const Range *ranges;         // range records of the multi-range items
+    const unsigned long long *overflow;  // planner's capacity flag
     const Block *blocks;
     const int64_t *n_items;      // device: work items in the list
     int n_waves;
@@ -972,7 +1253,7 @@ __device__ __forceinline__ void one_pair(const PairArgs &P, const SampleView &S,
 }
 
 template <int MODE, int SH, bool WEIGHTED, bool VAR>
-__device__ __forceinline__ void run_item(const PairArgs &P, const SampleView &S, int p0, int np,
+__device__ __forceinline__ void run_item(const PairArgs &P, const SampleView &S, int p0, int np, int nr,
                                          int shifts, ShapePtr shp, int ns, char *hist,
                                          const double *T, const double *X, int lane,
                                          uint32_t &nre, uint64_t &iterations) {
@@ -992,11 +1273,36 @@ __device__ __forceinline__ void run_item(const PairArgs &P, const SampleView &S,
     // pairs spread over more (r, pi) bins (fewer same-address LDS atomics).
     const int nb = (np + kWave - 1) / kWave;
     iterations += (uint64_t)nb * ns;
+    // The lane's current range: item coordinate k < ce is position cb + k.
+    // Lane l walks k = l*nb .. l*nb + nb - 1, so it starts in the range
+    // holding l*nb (found from the item's range records, scalar loads live
+    // only here) and steps to the next record when k reaches ce.
+    int cb = p0, ce = np, ci = 0;
+    const int2 *rg = (const int2 *)(P.ranges + (nr > 1 ? p0 : 0));
+    if (nr > 1) {
+        const MIA_CONST int32_t *rr = (const MIA_CONST int32_t *)rg;
+        const int kf = lane * nb < np ? lane * nb : np - 1;
+        cb = rr[0];
+        ce = rr[1];
+#pragma unroll
+        for (int i = 1; i < kItemRanges; ++i)
+            if (i < nr && kf >= rr[2 * i - 1]) {
+                ci = i;
+                cb = rr[2 * i];
+                ce = rr[2 * i + 1];
+            }
+    }
     Pend pd{-1, 0, 0, 0, 0, 0};
     for (int bt = 0; bt < nb; ++bt) {
-        int pi0 = bt + lane * nb;
-        bool v0 = pi0 < np;
-        int p0i = p0 + (v0 ? pi0 : 0);
+        const int k = bt + lane * nb;
+        const bool v0 = k < np;
+        while (v0 && k >= ce) {
+            const int2 r = rg[++ci];
+            cb = r.x;
+            ce = r.y;
+        }
+        // a lane past the item's end reads a valid position (its range's last)
+        const int p0i = cb + (v0 ? k : ce - 1);
         const double2 *pr = (const double2 *)(S.prec + 4 * (int64_t)p0i);
         const double2 ab = pr[0], lw = pr[1];
         double pa0 = ab.x, pb0 = ab.y, pl0 = lw.x;
@@ -1053,7 +1359,8 @@ __global__ __launch_bounds__(kThreads) void k_pairs(PairArgs P) {
     // position cells) stay on one XCD
     const int w = xcd_run_block() * (blockDim.x / kWave) + wv;
     const int64_t n_items = *(const MIA_CONST int64_t *)P.n_items;
-    if (w >= P.n_waves || n_items == 0) return;
+    // a plan that overflowed its capacities is counted again by the host
+    if (w >= P.n_waves || n_items == 0 || *(const MIA_CONST unsigned long long *)P.overflow) return;
     if (lane == 0) atomicMax(&P.clock[0], ~(unsigned long long)wall_clock64());
     // a contiguous share of the cost-sorted items, then tail chunks from the counter
     const WaveSchedule sch = make_schedule(P.bounds, P.n_waves);
@@ -1073,6 +1380,7 @@ __global__ __launch_bounds__(kThreads) void k_pairs(PairArgs P) {
             const int blk = ip[0], p0 = ip[1], np = ip[2], w3 = ip[3];
             const int rp = (int16_t)(w3 & 0xffff);
             const int shifts = (w3 >> 16) & 0xff;
+            const int nr = (w3 >> 24) & 0xff;
             const MIA_CONST int32_t *bp = blocks + 6 * (int64_t)blk;
             const int s0 = bp[0], ns = bp[1], sample = bp[4], rs = bp[5];
             if (sample != cur_key_s || rs != cur_rs || rp != cur_rp) {
@@ -1094,13 +1402,13 @@ __global__ __launch_bounds__(kThreads) void k_pairs(PairArgs P) {
             }
             ShapePtr shp = (ShapePtr)(S.srec + 8 * (int64_t)s0);
             if (shifts & 64)
-                run_item<MODE, SH_WRAP, WEIGHTED, VAR>(P, S, p0, np, shifts, shp, ns, hist, T, X, lane, nre,
+                run_item<MODE, SH_WRAP, WEIGHTED, VAR>(P, S, p0, np, nr, shifts, shp, ns, hist, T, X, lane, nre,
                                                        iterations);
             else if (shifts & 63)
-                run_item<MODE, SH_CONST, WEIGHTED, VAR>(P, S, p0, np, shifts, shp, ns, hist, T, X, lane, nre,
+                run_item<MODE, SH_CONST, WEIGHTED, VAR>(P, S, p0, np, nr, shifts, shp, ns, hist, T, X, lane, nre,
                                                         iterations);
             else
-                run_item<MODE, SH_NONE, WEIGHTED, VAR>(P, S, p0, np, shifts, shp, ns, hist, T, X, lane, nre,
+                run_item<MODE, SH_NONE, WEIGHTED, VAR>(P, S, p0, np, nr, shifts, shp, ns, hist, T, X, lane, nre,
                                                        iterations);
         }
     } while (sch.next(P.queue, lane, it0, it1));
@@ -1334,7 +1642,7 @@ void pair_kernels_init() {
 struct CountPlan {
     int S = 0;
     int K1 = 1, plan_local = 0, bsz = kMaxBlockShapes;
-    int64_t ncells = 0, nthreads = 0, nkeys = 0, fin_total = 0, item_cap = 0;
+    int64_t ncells = 0, nthreads = 0, nthreads_r = 0, nkeys = 0, fin_total = 0, item_cap = 0, range_cap = 0;
     int64_t acc_total = 0;           // accumulator words (u64 / f64) of all samples
     double margin = 0;               // largest geometric slack of the samples' grids
     PairConfig pc;
@@ -1345,13 +1653,14 @@ struct CountPlan {
     std::vector<PlanSample> ps;
     std::vector<SampleDev> sd;
     std::vector<FinSample> fs;
-    std::vector<int64_t> cell0, t0, fin0;
+    std::vector<int64_t> cell0, t0, t0r, fin0;
     std::vector<int64_t> acc0;       // first accumulator word of each sample
     std::vector<int> Bs;             // per sample: K
     // bound workspace
     char *param_dev = nullptr;
     std::vector<char> param;         // host image of the parameter block
-    size_t off_T = 0, off_X = 0, off_cell0 = 0, off_t0 = 0, off_fin0 = 0, off_ps = 0, off_sd = 0, off_fs = 0;
+    size_t off_T = 0, off_X = 0, off_cell0 = 0, off_t0 = 0, off_t0r = 0, off_fin0 = 0, off_ps = 0, off_sd = 0,
+           off_fs = 0;
     size_t param_bytes = 0;
     unsigned long long *zero_begin = nullptr;
     size_t zero_words = 0;
@@ -1360,8 +1669,12 @@ struct CountPlan {
     unsigned long long *kc = nullptr, *kcur = nullptr;
     int64_t *ko = nullptr;
     Item *items = nullptr;
+    Range *ranges = nullptr;
+    RawRange *raw = nullptr;
+    int32_t *roff = nullptr, *rcnt = nullptr;
     int64_t *cost = nullptr, *cpre = nullptr, *n_items = nullptr, *bounds = nullptr;
     double *bbox = nullptr;
+    int32_t *bruns = nullptr;
     unsigned long long *words = nullptr;   // [kCountReadback] read back after the count
     int *queue = nullptr;
     unsigned long long *acc = nullptr;
@@ -1385,6 +1698,7 @@ int count_bsz(const CountPlan &p) { return p.bsz; }
 bool count_det(const CountPlan &p) { return p.det; }
 int count_G(const CountPlan &p, int s) { return p.G[s]; }
 int64_t count_item_cap(const CountPlan &p) { return p.item_cap; }
+int64_t count_range_cap(const CountPlan &p) { return p.range_cap; }
 std::string count_key(const CountPlan &p) { return p.key; }
 size_t count_param_bytes(const CountPlan &p) { return p.param_bytes; }
 const void *count_param_image(const CountPlan &p) { return p.param.data(); }
@@ -1392,11 +1706,13 @@ void *count_param_dev(const CountPlan &p) { return p.param_dev; }
 const unsigned long long *count_readback(const CountPlan &p) { return p.words; }
 void free_plan(CountPlan *p) { delete p; }
 
-// Expected work items of a sample (mia_grid.hip's uniform-density model).
-double estimate_items(const Grid &g, int G, double reach_r, double reach_z, bool rmu, int64_t ns, int64_t np);
+// Expected work items and multi-range records of a sample (mia_grid.hip's
+// uniform-density model).
+double estimate_items(const Grid &g, int G, double reach_r, double reach_z, bool rmu, int64_t ns, int64_t np,
+                      int pmax, double *ranges);
 
 CountPlan *plan_count(Ctx &ctx, const std::vector<SortedSample> &ss, const mia_bins &bins,
-                      const mia_tuning *tuning, int64_t item_cap, bool cap_exact) {
+                      const mia_tuning *tuning, int64_t item_cap, int64_t range_cap, bool cap_exact) {
     pair_kernels_init();
     std::unique_ptr<CountPlan> P(new CountPlan());
     CountPlan &p = *P;
@@ -1426,9 +1742,10 @@ CountPlan *plan_count(Ctx &ctx, const std::vector<SortedSample> &ss, const mia_b
     p.G.resize(S);
     p.cell0.assign(S + 1, 0);
     p.t0.assign(S + 1, 0);
+    p.t0r.assign(S + 1, 0);
     p.fin0.assign(S + 1, 0);
     p.acc0.assign(S + 1, 0);
-    double est_items = 0;
+    double est_items = 0, est_ranges = 0;
     for (int si = 0; si < S; ++si) {
         const SortedSample &x = ss[si];
         const Grid &g = x.g;
@@ -1469,17 +1786,24 @@ CountPlan *plan_count(Ctx &ctx, const std::vector<SortedSample> &ss, const mia_b
         Q.cell0 = p.cell0[si];
         p.cell0[si + 1] = Q.cell0 + Q.ncells;
         const int64_t blk_cap = (x.n_shape + p.bsz - 1) / p.bsz + std::min<int64_t>(Q.ncells, x.n_shape);
-        Q.nt = blk_cap * Q.nxw * Q.nxw;
+        Q.ngc = (Q.nxw * Q.nxw + kGroupColumns - 1) / kGroupColumns;
+        Q.nt = blk_cap * Q.ngc;
         Q.t0 = p.t0[si];
         p.t0[si + 1] = Q.t0 + (Q.nt + kPlanThreads - 1) / kPlanThreads * kPlanThreads;
+        Q.ntr = blk_cap * Q.nxw * Q.nxw;
+        Q.t0r = p.t0r[si];
+        p.t0r[si + 1] = Q.t0r + (Q.ntr + kPlanThreads - 1) / kPlanThreads * kPlanThreads;
         const int K = x.K;
         p.fin0[si + 1] = p.fin0[si] + (int64_t)(1 + K) * B + K;
         // accumulators: npairs, sp, dd ((1+K) B each), sx, var (B each)
         p.acc0[si + 1] = p.acc0[si] + 3 * (int64_t)(1 + K) * B + 2 * (int64_t)B;
-        est_items += estimate_items(g, G, reach_r, reach_z, bins.mode == MIA_RMU, x.n_shape, x.n_pos);
+        double er = 0;
+        est_items += estimate_items(g, G, reach_r, reach_z, bins.mode == MIA_RMU, x.n_shape, x.n_pos, Q.pmax, &er);
+        est_ranges += er;
     }
     p.ncells = p.cell0[S];
     p.nthreads = p.t0[S];
+    p.nthreads_r = p.t0r[S];
     p.fin_total = p.fin0[S];
     p.acc_total = p.acc0[S];
     MIA_REQUIRE((double)S * p.K1 * p.K1 < 2e9, "too many planner keys (samples x regions^2)");
@@ -1492,6 +1816,8 @@ CountPlan *plan_count(Ctx &ctx, const std::vector<SortedSample> &ss, const mia_b
     // uniform-density estimate plus a margin
     p.item_cap = cap_exact ? item_cap : std::max<int64_t>(item_cap, (int64_t)(3.0 * est_items) + 65536);
     p.item_cap = std::min<int64_t>(p.item_cap, (int64_t)1 << 31);
+    p.range_cap = cap_exact ? range_cap : std::max<int64_t>(range_cap, (int64_t)(2.0 * est_ranges) + 65536);
+    p.range_cap = std::min<int64_t>(p.range_cap, ((int64_t)1 << 31) - kItemRanges);
 
     // the launch-sequence key: everything the launches depend on except the
     // caller's pointers (which live in the parameter block)
@@ -1503,6 +1829,7 @@ CountPlan *plan_count(Ctx &ctx, const std::vector<SortedSample> &ss, const mia_b
     put(p.X.data(), sizeof(double) * p.X.size());
     if (tuning) put(tuning, sizeof(*tuning));
     put(&p.item_cap, sizeof(p.item_cap));
+    put(&p.range_cap, sizeof(p.range_cap));
     for (int si = 0; si < S; ++si) {
         PlanSample q = p.ps[si];
         q.sstart = q.pstart = q.pfine = nullptr;
@@ -1541,6 +1868,7 @@ void count_bind(CountPlan &p, Carve &c, const std::vector<SortedSample> &ss, con
     p.off_X = pc.off; pc.take<double>(n_2 + 1);
     p.off_cell0 = pc.off; pc.take<int64_t>(S + 1);
     p.off_t0 = pc.off; pc.take<int64_t>(S + 1);
+    p.off_t0r = pc.off; pc.take<int64_t>(S + 1);
     p.off_fin0 = pc.off; pc.take<int64_t>(S + 1);
     p.off_ps = pc.off; pc.take<PlanSample>(S);
     p.off_sd = pc.off; pc.take<SampleDev>(S);
@@ -1562,12 +1890,18 @@ void count_bind(CountPlan &p, Carve &c, const std::vector<SortedSample> &ss, con
     p.boff = c.take<int32_t>(p.ncells + 1);
     scan_carve<int32_t>(c, p.ncells, p.scan_nb);
     int64_t blk_total = 0;
-    for (int si = 0; si < S; ++si) blk_total += p.ps[si].nt / (p.ps[si].nxw * p.ps[si].nxw);
+    for (int si = 0; si < S; ++si) blk_total += p.ps[si].nt / p.ps[si].ngc;
     p.blocks = c.take<Block>(std::max<int64_t>(blk_total, 1));
     p.bbox = c.take<double>(6 * std::max<int64_t>(blk_total, 1));
+    p.bruns = c.take<int32_t>((kBlockRuns + 1) * std::max<int64_t>(blk_total, 1));
     p.ko = c.take<int64_t>(p.nkeys + 1);
     scan_carve<int64_t>(c, p.nkeys, p.scan_keys);
     p.items = c.take<Item>(p.item_cap);
+    // kItemRanges records of slack: the pair kernel loads a full set per item
+    p.ranges = c.take<Range>(p.range_cap + kItemRanges);
+    p.raw = c.take<RawRange>(p.range_cap);
+    p.roff = c.take<int32_t>(std::max<int64_t>(p.nthreads_r, 1));
+    p.rcnt = c.take<int32_t>(std::max<int64_t>(p.nthreads_r, 1));
     p.cost = c.take<int64_t>(p.item_cap + 1);
     p.cpre = c.take<int64_t>(p.item_cap + 1);
     scan_carve<int64_t>(c, p.item_cap, p.scan_cost);
@@ -1610,6 +1944,7 @@ void count_bind(CountPlan &p, Carve &c, const std::vector<SortedSample> &ss, con
     std::memcpy(img + p.off_X, p.X.data(), sizeof(double) * (n_2 + 1));
     std::memcpy(img + p.off_cell0, p.cell0.data(), sizeof(int64_t) * (S + 1));
     std::memcpy(img + p.off_t0, p.t0.data(), sizeof(int64_t) * (S + 1));
+    std::memcpy(img + p.off_t0r, p.t0r.data(), sizeof(int64_t) * (S + 1));
     std::memcpy(img + p.off_fin0, p.fin0.data(), sizeof(int64_t) * (S + 1));
     unsigned long long *acc = p.acc;
     double *accd = (double *)p.acc;
@@ -1665,18 +2000,39 @@ void count_enqueue(const CountPlan &p, hipStream_t st, const hipEvent_t *ev) {
     const PlanSample *ps = (const PlanSample *)(pd + p.off_ps);
     const int64_t *cell0 = (const int64_t *)(pd + p.off_cell0);
     const int64_t *t0 = (const int64_t *)(pd + p.off_t0);
+    const int64_t *t0r = (const int64_t *)(pd + p.off_t0r);
     const int64_t *fin0 = (const int64_t *)(pd + p.off_fin0);
     if (ev) MIA_HIP(hipEventRecord(ev[0], st));
     fill<unsigned long long>(p.zero_begin, 0ull, (int64_t)p.zero_words, st);
     // shape blocks of every (column, slab group) cell
     hipLaunchKernelGGL(k_blocks, dim3(nblk(p.ncells, 256)), dim3(256), 0, st, ps, cell0, S, p.bsz, p.nb,
-                       (const int32_t *)nullptr, (Block *)nullptr, (double *)nullptr);
+                       (const int32_t *)nullptr, (Block *)nullptr, (double *)nullptr, (int32_t *)nullptr);
     MIA_HIP(hipGetLastError());
     scan_ws<int32_t>(p.nb, p.boff, p.ncells, nullptr, p.scan_nb, st);
     hipLaunchKernelGGL(k_blocks, dim3(nblk(p.ncells, 256)), dim3(256), 0, st, ps, cell0, S, p.bsz, p.nb,
-                       (const int32_t *)p.boff, p.blocks, p.bbox);
+                       (const int32_t *)p.boff, p.blocks, p.bbox, p.bruns);
     MIA_HIP(hipGetLastError());
-    // work items, counting-sorted by (sample, R_s, R_p)
+    // the position ranges of every (block, stencil column) pair
+    RangeLaunch R{};
+    R.ps = ps;
+    R.t0r = t0r;
+    R.S = S;
+    R.K1 = p.K1;
+    R.boff = p.boff;
+    R.blocks = p.blocks;
+    R.bbox = p.bbox;
+    R.bruns = p.bruns;
+    R.raw = p.raw;
+    R.raw_cursor = p.words + 8;
+    R.raw_cap = p.range_cap;
+    R.roff = p.roff;
+    R.rcnt = p.rcnt;
+    R.overflow = p.words + 2;
+    const int64_t nvwg_r = (p.nthreads_r + kPlanThreads - 1) / kPlanThreads;
+    const unsigned rgrid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(nvwg_r, 256 * kPlanGridPerCu));
+    hipLaunchKernelGGL(k_plan_ranges, dim3(rgrid), dim3(kPlanThreads), 0, st, R);
+    MIA_HIP(hipGetLastError());
+    // work items packed from the ranges, counting-sorted by (sample, R_s, R_p)
     PlanLaunch L{};
     L.ps = ps;
     L.t0 = t0;
@@ -1684,13 +2040,18 @@ void count_enqueue(const CountPlan &p, hipStream_t st, const hipEvent_t *ev) {
     L.K1 = p.K1;
     L.boff = p.boff;
     L.blocks = p.blocks;
-    L.bbox = p.bbox;
+    L.raw = p.raw;
+    L.roff = p.roff;
+    L.rcnt = p.rcnt;
     L.key_count = p.kc;
     L.key_cursor = p.kcur;
     L.key_off = p.ko;
     L.items = p.items;
     L.cost = p.cost;
     L.cap = p.item_cap;
+    L.ranges = p.ranges;
+    L.range_cursor = p.words + 7;
+    L.range_cap = p.range_cap;
     L.overflow = p.words + 2;
     L.candidates = p.words + 1;
     L.local = p.plan_local;
@@ -1746,6 +2107,8 @@ void count_enqueue(const CountPlan &p, hipStream_t st, const hipEvent_t *ev) {
     // pair kernel, one launch per r-bin pass
     PairArgs A = p.pc.A;
     A.items = items;
+    A.ranges = p.ranges;
+    A.overflow = p.words + 2;
     A.blocks = p.blocks;
     A.n_items = p.n_items;
     A.bounds = p.bounds;

@@ -15,6 +15,7 @@
 // (box, binning reach, jackknife, sizes).
 #include "mia_common.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <mutex>
 #include <tuple>
@@ -40,9 +41,14 @@ struct Choice {
     int ncj, nzj, G, pmax;
 };
 
-// Expected candidates and work items of one grid (uniform densities).
+// Expected candidates, work items and position ranges of one grid (uniform
+// densities).  Per shape block: every reached stencil column holds
+// `pos` positions in (1 + win/jspan + win/L) line-of-sight runs (jackknife
+// slab and periodic-wrap splits); the planner packs the runs of each group of
+// kGroupColumns columns into items of at most pmax positions and kItemRanges
+// runs, one item per (region, shift) class at least.
 void estimate(double L, int jk, int ncj, int nzj, int G, double R, double Z, bool rmu, double Ns,
-              double Np, double &cand, double &items, int pmax = kItemPositions) {
+              double Np, double &cand, double &items, double *ranges = nullptr, int pmax = kItemPositions) {
     const int nc = ncj * jk, nz = nzj * jk;
     const double cw = L / nc, hz = L / nz;
     int kx = (int)std::ceil(R / cw) + 1;
@@ -51,14 +57,30 @@ void estimate(double L, int jk, int ncj, int nzj, int G, double R, double Z, boo
         lo = -(nc / 2);
         hi = lo + nc - 1;
     }
+    const int nxw = hi - lo + 1;
     const double cell_p = Np / ((double)nc * nc * nz);
     const double jspan = L / jk;
-    double zsum = 0.0, runs = 0.0;
+    double zsum = 0.0, per_block = 0.0, rec = 0.0;
+    double g_pos = 0.0, g_runs = 0.0, g_cls = 0.0;
+    int g_at = -1;
+    auto close_group = [&]() {
+        if (g_cls > 0.0) {
+            const double n = std::max({g_cls, std::ceil(g_pos / pmax) + g_cls - 1.0, std::ceil(g_runs / kItemRanges)});
+            per_block += n;
+            rec += g_runs;
+        }
+        g_pos = g_runs = g_cls = 0.0;
+    };
     for (int i = lo; i <= hi; ++i) {
         int ai = std::abs(i);
         if (2 * kx + 1 >= nc) ai = std::min(ai, nc - ai);
         double u = std::max(ai - 1, 0) * cw;
         for (int j = lo; j <= hi; ++j) {
+            const int grp = ((i - lo) * nxw + (j - lo)) / kGroupColumns;
+            if (grp != g_at) {
+                close_group();
+                g_at = grp;
+            }
             int aj = std::abs(j);
             if (2 * kx + 1 >= nc) aj = std::min(aj, nc - aj);
             double v = std::max(aj - 1, 0) * cw;
@@ -69,14 +91,19 @@ void estimate(double L, int jk, int ncj, int nzj, int G, double R, double Z, boo
             zsum += nsl;
             double win = nsl * hz;
             double pos = cell_p * nsl;
-            runs += (1.0 + win / jspan + win / L) * std::max(1.0, std::ceil(pos / pmax));
+            const double cls = 1.0 + win / jspan + win / L;
+            g_pos += pos;
+            g_runs += cls * std::max(1.0, std::ceil(pos / pmax));
+            g_cls = std::max(g_cls, cls);
         }
     }
+    close_group();
     cand = Ns * cell_p * zsum;
     const double groups = (double)nc * nc * (nz / G);
     const double lam = Ns / groups;
     const double nblocks = groups * (1.0 - std::exp(-lam)) * std::max(1.0, lam / kBlockShapes);
-    items = kItemScale * nblocks * runs;
+    items = kItemScale * nblocks * per_block;
+    if (ranges) *ranges = nblocks * rec;
 }
 
 std::mutex g_mu;
@@ -116,7 +143,7 @@ Choice search(double L, int jk, double R, double Z, bool rmu, int64_t ns, int64_
     {
         double cand, items, items2;
         estimate(L, jk, best.ncj, best.nzj, best.G, R, Z, rmu, Ns, Np, cand, items);
-        estimate(L, jk, best.ncj, best.nzj, best.G, R, Z, rmu, Ns, Np, cand, items2, kItemPositionsDense);
+        estimate(L, jk, best.ncj, best.nzj, best.G, R, Z, rmu, Ns, Np, cand, items2, nullptr, kItemPositionsDense);
         if (items2 < 0.9 * items) best.pmax = kItemPositionsDense;
     }
     std::lock_guard<std::mutex> lk(g_mu);
@@ -126,12 +153,15 @@ Choice search(double L, int jk, double R, double Z, bool rmu, int64_t ns, int64_
 
 }  // namespace
 
-double estimate_items(const Grid &g, int G, double reach_r, double reach_z, bool rmu, int64_t ns, int64_t np) {
+double estimate_items(const Grid &g, int G, double reach_r, double reach_z, bool rmu, int64_t ns, int64_t np,
+                      int pmax, double *ranges) {
     const int jk = g.jk > 0 ? g.jk : 1;
-    double cand = 0, items = 0;
+    double cand = 0, items = 0, rec = 0;
     estimate(g.L, jk, g.ncj, g.nzj, std::max(1, G), reach_r, reach_z, rmu, (double)std::max<int64_t>(ns, 1),
-             (double)std::max<int64_t>(np, 1), cand, items);
-    return ns > 0 && np > 0 ? items : 0.0;
+             (double)std::max<int64_t>(np, 1), cand, items, &rec, pmax);
+    const bool any = ns > 0 && np > 0;
+    if (ranges) *ranges = any ? rec : 0.0;
+    return any ? items : 0.0;
 }
 
 Grid make_grid(const mia_sample &s, const mia_bins &bins, const mia_tuning *t) {

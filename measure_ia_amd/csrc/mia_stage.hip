@@ -846,6 +846,18 @@ void stage_release(Ctx &ctx) {
     ctx.ws_bytes = ctx.pinned_bytes = 0;
 }
 
+// After a count whose plan overflowed: larger item and range capacities for
+// the next attempt (a quarter of headroom over what the count needed).
+void grow_caps(Ctx &ctx, const RunOut &r, int attempt) {
+    if (attempt >= 3)
+        throw Error{MIA_E_UNSUPPORTED, "work items exceed the workspace capacity (" + std::to_string(r.words[3]) +
+                                           " items, " + std::to_string(std::max(r.words[7], r.words[8])) +
+                                           " range records)"};
+    ctx.item_cap = std::max<int64_t>(ctx.item_cap, (int64_t)(r.words[3] + r.words[3] / 4 + 1024));
+    const unsigned long long need = std::max(r.words[7], r.words[8]);
+    ctx.range_cap = std::max<int64_t>(ctx.range_cap, (int64_t)(need + need / 4 + 1024));
+}
+
 void stage_count(Ctx &ctx, int n, const mia_sample *samples, const mia_bins &bins, const mia_counts *outs,
                  const mia_tuning *tuning, mia_stats *stats, hipStream_t st, StageMode mode) {
     // the deterministic sort's radix-sort launches stay out of graphs
@@ -856,7 +868,7 @@ void stage_count(Ctx &ctx, int n, const mia_sample *samples, const mia_bins &bin
         Carve dry;
         std::vector<SortedSample> proto = prep_bind(prep, dry, samples);
         const size_t pbytes = dry.off;
-        PlanPtr cp(plan_count(ctx, proto, bins, tuning, ctx.item_cap, false));
+        PlanPtr cp(plan_count(ctx, proto, bins, tuning, ctx.item_cap, ctx.range_cap, false));
         size_t total = pbytes + count_bytes(*cp);
         char *base;
         if (mode == STAGE_STATIC) {
@@ -864,11 +876,12 @@ void stage_count(Ctx &ctx, int n, const mia_sample *samples, const mia_bins &bin
             const size_t room = (size_t)stage_static_bytes();
             if (total > room) {
                 // fewer work-item slots (32 bytes each with their costs and
-                // scan), down to a floor; a count that needs more reports it
-                const int64_t over = (int64_t)((total - room) / 32) + 4096;
-                const int64_t cap = count_item_cap(*cp) - over;
-                if (cap >= 65536) {
-                    cp.reset(plan_count(ctx, proto, bins, tuning, cap, true));
+                // scan) and ranges (24 bytes: planner range and item record),
+                // down to a floor; a count that needs more reports it
+                const int64_t over = (int64_t)((total - room) / 56) + 4096;
+                const int64_t cap = count_item_cap(*cp) - over, rcap = count_range_cap(*cp) - over;
+                if (cap >= 65536 && rcap >= 65536) {
+                    cp.reset(plan_count(ctx, proto, bins, tuning, cap, rcap, true));
                     total = pbytes + count_bytes(*cp);
                 }
             }
@@ -891,16 +904,16 @@ void stage_count(Ctx &ctx, int n, const mia_sample *samples, const mia_bins &bin
             if (r.oor[s])
                 throw Error{MIA_E_RANGE, "coordinates outside [0, boxsize] (or NaN) in sample " + std::to_string(s)};
         if (r.words[2]) {
-            // more work items than the capacity: grow it and count again
+            // more work items or range records than the capacity: grow it and count again
             if (mode == STAGE_STATIC)
                 throw Error{MIA_E_UNSUPPORTED, "the count needs " + std::to_string(r.words[3]) +
-                                                   " work items; the static workspace holds " +
-                                                   std::to_string(count_item_cap(*cp)) +
+                                                   " work items and " +
+                                                   std::to_string(std::max(r.words[7], r.words[8])) +
+                                                   " range records; the static workspace holds " +
+                                                   std::to_string(count_item_cap(*cp)) + " and " +
+                                                   std::to_string(count_range_cap(*cp)) +
                                                    " (mia_count allocates what it needs)"};
-            if (attempt >= 3)
-                throw Error{MIA_E_UNSUPPORTED, "work items exceed the workspace capacity (" +
-                                                   std::to_string(r.words[3]) + " items)"};
-            ctx.item_cap = std::max<int64_t>(ctx.item_cap, (int64_t)(r.words[3] + r.words[3] / 4 + 1024));
+            grow_caps(ctx, r, attempt);
             continue;
         }
         fill_stats(stats, r, *cp, prep.grids[0]);
@@ -950,7 +963,7 @@ void stage_count_prepared(Ctx &ctx, int n, const PreparedSample *const *hs, cons
         ss[i] = hs[i]->s;
     }
     for (int attempt = 0;; ++attempt) {
-        PlanPtr cp(plan_count(ctx, ss, bins, tuning, ctx.item_cap, false));
+        PlanPtr cp(plan_count(ctx, ss, bins, tuning, ctx.item_cap, ctx.range_cap, false));
         Carve c;
         c.dry = false;
         c.base = (char *)workspace(ctx, count_bytes(*cp));
@@ -958,10 +971,7 @@ void stage_count_prepared(Ctx &ctx, int n, const PreparedSample *const *hs, cons
         RunOut r;
         upload_and_run(ctx, nullptr, *cp, st, STAGE_EAGER, r);
         if (r.words[2]) {
-            if (attempt >= 3)
-                throw Error{MIA_E_UNSUPPORTED, "work items exceed the workspace capacity (" +
-                                                   std::to_string(r.words[3]) + " items)"};
-            ctx.item_cap = std::max<int64_t>(ctx.item_cap, (int64_t)(r.words[3] + r.words[3] / 4 + 1024));
+            grow_caps(ctx, r, attempt);
             continue;
         }
         fill_stats(stats, r, *cp, ss[0].g);
