@@ -1,18 +1,20 @@
 // Grid autotuner: columns per side, slabs per side and slabs per shape block.
 //
-// The pair kernel's time is, to within a few percent over the measured range,
-//     t = a * candidates + b * items + c * pairs
-// (candidates = (shape, position) pairs the work items evaluate, items = work
-// items, pairs = in-domain pairs, fixed by the data).  For a grid the first two
-// are estimated on the host for uniform densities -- the expected number of
-// positions in the cells each shape block reaches, and the blocks x reached
-// columns x line-of-sight runs -- and the grid with the lowest a*cand + b*items
-// is taken.  The candidate estimate matches clustered samples to <1 %; the
-// item estimate is scaled by its measured ratio (0.75).  a and b were fitted
-// on MI355X sweeps of a 1M-position w_gg/w_g+ count and a 2M-position
-// multipole count: a = 1.1 ps (r_p, pi) and 1.9 ps (r, mu) per candidate,
-// b = 0.5 ns (r_p, pi) and 1.3 ns (r, mu) per item (pair kernel + planning).  Choices are cached per
-// (box, binning reach, jackknife, sizes).
+// The count's time (pair kernel + planning) is, to within a few percent over
+// the measured range,
+//     t = t0 + a * candidates + b * items
+// (t0: the in-domain pairs, fixed by the data; candidates = (shape, position)
+// pairs the work items evaluate; items = packed work items).  For a grid the
+// last two are estimated on the host for uniform densities -- the expected
+// number of positions in the cells each shape block reaches, and the items the
+// planner packs from each group of stencil columns -- and the grid with the
+// lowest a*cand + b*items is taken.  The candidate estimate is ~10 % high on
+// clustered samples (the same for every grid); the item estimate is scaled by
+// its measured ratio (0.7).  a and b were fitted on MI355X sweeps of a
+// 1M-position w_gg/w_g+ count and a 2M-position multipole count: a = 0.58 ps
+// (r_p, pi) and 0.85 ps (r, mu) per candidate, b = 1.0 ns (r_p, pi) and
+// 1.3 ns (r, mu) per item.  Choices are cached per (box, binning reach,
+// jackknife, sizes).
 #include "mia_common.hpp"
 
 #include <algorithm>
@@ -24,11 +26,11 @@ namespace mia {
 
 namespace {
 
-constexpr double kCandCostW = 1.1e-12;   // s per candidate, (r_p, pi)
-constexpr double kCandCostMu = 1.9e-12;  // s per candidate, (r, mu)
-constexpr double kItemCostW = 5e-10;     // s per work item, (r_p, pi)
+constexpr double kCandCostW = 0.58e-12;  // s per candidate, (r_p, pi)
+constexpr double kCandCostMu = 0.85e-12; // s per candidate, (r, mu)
+constexpr double kItemCostW = 1.0e-9;    // s per work item, (r_p, pi)
 constexpr double kItemCostMu = 1.3e-9;   // s per work item, (r, mu)
-constexpr double kItemScale = 0.75;      // measured items / estimated items
+constexpr double kItemScale = 0.7;       // measured items / estimated items
 constexpr int kBlockShapes = 64;
 constexpr int kItemPositions = 1024;
 // Dense columns split their position runs into many 1024-position items;
