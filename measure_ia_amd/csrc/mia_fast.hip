@@ -1292,19 +1292,31 @@ __device__ __forceinline__ void run_item(const PairArgs &P, const SampleView &S,
                 ce = rr[2 * i + 1];
             }
     }
-    Pend pd{-1, 0, 0, 0, 0, 0};
-    for (int bt = 0; bt < nb; ++bt) {
+    // the lane's position of batch bt (a lane past the item's end reads a
+    // valid position, its range's last)
+    auto position = [&](int bt, bool &v) {
         const int k = bt + lane * nb;
-        const bool v0 = k < np;
-        while (v0 && k >= ce) {
+        v = k < np;
+        while (v && k >= ce) {
             const int2 r = rg[++ci];
             cb = r.x;
             ce = r.y;
         }
-        // a lane past the item's end reads a valid position (its range's last)
-        const int p0i = cb + (v0 ? k : ce - 1);
-        const double2 *pr = (const double2 *)(S.prec + 4 * (int64_t)p0i);
-        const double2 ab = pr[0], lw = pr[1];
+        return (const double2 *)(S.prec + 4 * (int64_t)(cb + (v ? k : ce - 1)));
+    };
+    Pend pd{-1, 0, 0, 0, 0, 0};
+    // positions are loaded one batch ahead
+    bool vn;
+    const double2 *prn = position(0, vn);
+    double2 abn = prn[0], lwn = prn[1];
+    for (int bt = 0; bt < nb; ++bt) {
+        const bool v0 = vn;
+        const double2 ab = abn, lw = lwn;
+        if (bt + 1 < nb) {
+            prn = position(bt + 1, vn);
+            abn = prn[0];
+            lwn = prn[1];
+        }
         double pa0 = ab.x, pb0 = ab.y, pl0 = lw.x;
         double pw0 = WEIGHTED ? lw.y : 1.0;
         // two shapes per iteration, the record after them loaded ahead (scalar
