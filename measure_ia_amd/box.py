@@ -229,11 +229,30 @@ class MeasureIABox(SimInfo):
                 cache = self._cache
         if cache is None:
             self._cache = None  # releases the previous device copy and sorts
+            upload = None
+            if reuse:
+                # the positions (the largest array, used as given) go to the
+                # device while the host prepares the shape columns
+                eng, dev_pos = self._engine(), {}
+                src = self.data["Position"]
+
+                def send():
+                    try:
+                        dev_pos["positions"] = eng.to_device({"positions": src})["positions"]
+                    except BaseException as exc:  # re-raised by the caller
+                        dev_pos["error"] = exc
+
+                upload = threading.Thread(target=send, daemon=True)
+                upload.start()
             p = hostprep.prepare(self.data, masks, ellipticity)
             host = dict(positions=p.positions, shapes=p.shapes, axis=p.axis, e=p.e,
                         weight=p.weight, weight_shape=p.weight_shape)
             if reuse:
-                host = self._engine().to_device(host)
+                rest = self._engine().to_device({k: v for k, v in host.items() if k != "positions"})
+                upload.join()
+                if "error" in dev_pos:
+                    raise dev_pos["error"]
+                host = dict(rest, positions=dev_pos["positions"])
                 cache = self._cache = {"key": key, "p": p, "arrays": host, "sorted": {}}
                 if key is None:
                     # data seen for the first time: fingerprint it in a thread
@@ -279,9 +298,7 @@ class MeasureIABox(SimInfo):
             counts["SplusD_jk"] = res.splus[1:]
             rs = res.region_stats
             region_counts = rs[:, :2]
-            tot_a = hostprep.sequential_sum(p.weight_shape * (1 - p.e ** 2 / 2.0))
-            tot_w = hostprep.sequential_sum(p.weight_shape)
-            region_R = estimators.responsivity_jk(tot_a, tot_w, rs[:, 3], rs[:, 2])
+            region_R = estimators.responsivity_jk(p.sum_wa, p.sum_w, rs[:, 3], rs[:, 2])
         var = None
         if variance and res.variance is not None:
             var = res.variance * inv2R * inv2R

@@ -12,7 +12,7 @@ from typing import Optional, Sequence
 
 import numpy as np
 
-from . import _capi
+from . import _capi, hostprep
 from .binning import Thresholds
 
 METHODS = {"plain": _capi.MIA_PLAIN, "fast": _capi.MIA_FAST}
@@ -76,7 +76,7 @@ def _unit(w) -> bool:
     Host arrays are checked on the host.  A device tensor is taken as given:
     checking it would cost a reduction and a sync per count, and a weighted
     count of unit weights gives the same DD (each product is exactly 1)."""
-    if w is None:
+    if w is None or hostprep.is_unit(w):
         return True
     try:
         import torch

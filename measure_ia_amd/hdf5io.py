@@ -148,6 +148,10 @@ class H5Store:
         # back by read() without a file round trip; the file holds the same)
         self._groups = {""}
         self._written = {}
+        # groups this store created (a dataset in one exists only if this
+        # store wrote it) and the dataspaces of the shapes it wrote
+        self._fresh = set()
+        self._spaces = {}
         lib = _load()
         if lib is None:
             try:
@@ -180,6 +184,9 @@ class H5Store:
             self._h5py = None
             return
         if getattr(self, "_fid", -1) >= 0:
+            for sid in self._spaces.values():
+                self._lib.H5Sclose(sid)
+            self._spaces = {}
             self._lib.H5Fclose(self._fid)
             self._fid = -1
 
@@ -278,6 +285,7 @@ class H5Store:
             if gid < 0:
                 raise OSError(f"cannot create group {sub!r}")
             self._lib.H5Gclose(gid)
+            self._fresh.add(sub)
         for i in range(1, len(parts) + 1):
             self._groups.add("/".join(parts[:i]))
         return path
@@ -293,6 +301,7 @@ class H5Store:
             arr = np.ascontiguousarray(arr, dtype=np.int64)
         else:
             arr = np.ascontiguousarray(arr, dtype=np.float64)
+        seen = path in self._written
         self._written[path] = arr.copy()
         if self._h5py is not None:
             if path in self._h5py:
@@ -300,26 +309,27 @@ class H5Store:
             self._h5py.create_dataset(path, data=arr)
             return
         lib = self._lib
-        if lib.H5Lexists(self._fid, path.encode(), _H5P_DEFAULT) > 0:
-            if lib.H5Ldelete(self._fid, path.encode(), _H5P_DEFAULT) < 0:
+        bpath = path.encode()
+        if (seen or group not in self._fresh) and lib.H5Lexists(self._fid, bpath, _H5P_DEFAULT) > 0:
+            if lib.H5Ldelete(self._fid, bpath, _H5P_DEFAULT) < 0:
                 raise OSError(f"cannot delete {path!r}")
-        if arr.ndim == 0:
-            sid = lib.H5Screate(0)  # H5S_SCALAR
-        else:
-            dims = (_hsize * arr.ndim)(*arr.shape)
-            sid = lib.H5Screate_simple(arr.ndim, dims, None)
-        if sid < 0:
-            raise OSError("H5Screate failed")
+        sid = self._spaces.get(arr.shape)
+        if sid is None:
+            if arr.ndim == 0:
+                sid = lib.H5Screate(0)  # H5S_SCALAR
+            else:
+                dims = (_hsize * arr.ndim)(*arr.shape)
+                sid = lib.H5Screate_simple(arr.ndim, dims, None)
+            if sid < 0:
+                raise OSError("H5Screate failed")
+            self._spaces[arr.shape] = sid
         mtype = lib._H5T_NATIVE_INT64_g.value if arr.dtype == np.int64 else lib._H5T_NATIVE_DOUBLE_g.value
-        did = lib.H5Dcreate2(self._fid, path.encode(), mtype, sid, _H5P_DEFAULT, _H5P_DEFAULT,
-                             _H5P_DEFAULT)
+        did = lib.H5Dcreate2(self._fid, bpath, mtype, sid, _H5P_DEFAULT, _H5P_DEFAULT, _H5P_DEFAULT)
         if did < 0:
-            lib.H5Sclose(sid)
             raise OSError(f"cannot create dataset {path!r}")
         rc = lib.H5Dwrite(did, mtype, _H5S_ALL, _H5S_ALL, _H5P_DEFAULT,
                           arr.ctypes.data_as(ctypes.c_void_p))
         lib.H5Dclose(did)
-        lib.H5Sclose(sid)
         if rc < 0:
             raise OSError(f"cannot write dataset {path!r}")
 

@@ -25,6 +25,9 @@ class Prepared:
     R: float
     los: int
     not_los: np.ndarray
+    # the two sums of R, sum(w_s (1 - e^2/2)) and sum(w_s), kept for R_jk
+    sum_wa: float = 0.0
+    sum_w: float = 0.0
 
     @property
     def n_pos(self):
@@ -33,6 +36,16 @@ class Prepared:
     @property
     def n_shape(self):
         return len(self.shapes)
+
+
+def unit_weights(n: int) -> np.ndarray:
+    """n weights of 1.0 as a read-only view of a single value."""
+    return np.broadcast_to(np.float64(1.0), (n,))
+
+
+def is_unit(w) -> bool:
+    """Whether w is a view made by unit_weights (no scan needed)."""
+    return isinstance(w, np.ndarray) and w.ndim == 1 and w.strides == (0,) and (w.size == 0 or w[0] == 1.0)
 
 
 def sequential_sum(a: np.ndarray) -> float:
@@ -54,8 +67,10 @@ def prepare(data: dict, masks=None, ellipticity: str = "distortion") -> Prepared
     """
     n_pos = len(data["Position"])
     n_shape = len(data["Position_shape_sample"])
-    w_all = data["weight"] if "weight" in data else np.ones(n_pos)
-    ws_all = data["weight_shape_sample"] if "weight_shape_sample" in data else np.ones(n_shape)
+    # absent weights are ones: read-only views of one value, which the engine
+    # recognises without scanning them
+    w_all = data["weight"] if "weight" in data else unit_weights(n_pos)
+    ws_all = data["weight_shape_sample"] if "weight_shape_sample" in data else unit_weights(n_shape)
     if masks is None:
         pos = data["Position"]
         shp = data["Position_shape_sample"]
@@ -87,13 +102,16 @@ def prepare(data: dict, masks=None, ellipticity: str = "distortion") -> Prepared
     else:
         raise ValueError("Invalid value for ellipticity. Choose 'distortion' or 'ellipticity'.")
     ws = np.asarray(ws, dtype=np.float64)
-    R = sequential_sum(ws * (1 - e ** 2 / 2.0)) / sequential_sum(ws)
+    sum_wa = sequential_sum(ws * (1 - e ** 2 / 2.0))
+    sum_w = float(len(ws)) if is_unit(ws) else sequential_sum(ws)
+    R = sum_wa / sum_w
     los = int(data["LOS"])
     not_los = np.array([c for c in range(3) if c != los])
+    w = w if is_unit(w) else np.ascontiguousarray(w, dtype=np.float64)
     return Prepared(np.ascontiguousarray(pos, dtype=np.float64),
                     np.ascontiguousarray(shp, dtype=np.float64),
                     np.ascontiguousarray(axis, dtype=np.float64), np.ascontiguousarray(e),
-                    np.ascontiguousarray(w, dtype=np.float64), ws, R, los, not_los)
+                    w, ws, R, los, not_los, sum_wa, sum_w)
 
 
 def jackknife_regions(xyz: np.ndarray, boxsize: float, L_subboxes: int) -> np.ndarray:
