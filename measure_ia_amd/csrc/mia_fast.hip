@@ -1825,10 +1825,12 @@ CountPlan *plan_count(Ctx &ctx, const std::vector<SortedSample> &ss, const mia_b
     const size_t plan_lds = (size_t)p.K1 * p.K1 * (sizeof(int64_t) + sizeof(int32_t));
     p.plan_local = plan_lds <= 64 * 1024 ? 1 : 0;
     // item capacity: the caller's (grown after an overflow), at least 3x the
-    // uniform-density estimate plus a margin
+    // uniform-density estimate plus a margin; range capacity (the planner's
+    // ranges and the items' range records): 1.25x the estimate, which runs
+    // 1.3-2x above the counts of the clustered benchmark samples
     p.item_cap = cap_exact ? item_cap : std::max<int64_t>(item_cap, (int64_t)(3.0 * est_items) + 65536);
     p.item_cap = std::min<int64_t>(p.item_cap, (int64_t)1 << 31);
-    p.range_cap = cap_exact ? range_cap : std::max<int64_t>(range_cap, (int64_t)(2.0 * est_ranges) + 65536);
+    p.range_cap = cap_exact ? range_cap : std::max<int64_t>(range_cap, (int64_t)(1.25 * est_ranges) + 65536);
     p.range_cap = std::min<int64_t>(p.range_cap, ((int64_t)1 << 31) - kItemRanges);
 
     // the launch-sequence key: everything the launches depend on except the
