@@ -36,15 +36,26 @@ from .siminfo import SimInfo
 BACKENDS = {"hip": "fast", "hip-plain": "plain"}
 
 
+_HASH_CHUNK = 1 << 20
+
+
 def _fingerprint(data) -> tuple:
     """Content key of a data dict: every array's dtype, shape and a 64-bit
     hash of its bytes (xxh3, or blake2b where xxhash is missing); scalars by
-    value.  Two calls on unchanged data get the same key."""
+    value.  Two calls on unchanged data get the same key.
+
+    The bytes are hashed in 1 MiB pieces with a yield of the interpreter lock
+    after each (xxh3 holds it while it hashes), so a count running on another
+    thread is not held up for the whole hash."""
     try:
         import xxhash
 
         def h(b):
-            return xxhash.xxh3_64_intdigest(b)
+            x = xxhash.xxh3_64()
+            for i in range(0, len(b), _HASH_CHUNK):
+                x.update(b[i:i + _HASH_CHUNK])
+                time.sleep(0)
+            return x.intdigest()
     except ImportError:  # pragma: no cover
         import hashlib
 
