@@ -6,9 +6,11 @@
 //   1. plan   : shapes of one column and a few line-of-sight slabs form a
 //               block (<= 64 shapes, one jackknife region R_s); every block
 //               is paired with the position cell ranges within reach, split
-//               at jackknife sub-box faces so that every work item has one
-//               position region R_p and one periodic shift per axis.  Items
-//               are counting-sorted by (sample, R_s, R_p).
+//               at jackknife sub-box faces so that every range has one
+//               position region R_p and one periodic shift per axis.  The
+//               ranges of up to 8 neighbouring stencil columns with the same
+//               region and shift are packed into one work item.  Items are
+//               counting-sorted by (sample, R_s, R_p).
 //   2. pairs  : a persistent grid of wavefronts takes contiguous runs of the
 //               sorted list, balanced by estimated cost, then claims tail
 //               chunks from a counter.  Each wave reads its
