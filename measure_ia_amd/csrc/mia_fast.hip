@@ -191,23 +191,30 @@ __device__ __forceinline__ double shift_val(int cls, double L) {
 // [zlo, zhi]) for a column at squared transverse distance dxy2, merged into
 // runs [k0, k) of equal periodic shift class and jackknife slab:
 // close(k0, k, class) for each run in slab order.  For (r_p, pi) the runs do
-// not depend on the column.
+// not depend on the column; for (r, mu) a column's runs are those of
+// dxy2 = 0 with slabs out of its reach taken off their ends.
+// Whether slab k is within reach of a block's shapes (extent [zlo, zhi]) for
+// a column at squared transverse distance dxy2, and its shift class cz.
+__device__ __forceinline__ bool slab_in(const PlanSample &P, double zlo, double zhi, int k, double dxy2, double rr,
+                                        int &cz) {
+    const Grid &g = P.g;
+    const double m = g.margin;
+    double dzmin;
+    axis_class(zlo - (k + 1) * g.hz - 2 * m, zhi - k * g.hz + 2 * m, g.L, g.half, P.periodic, dzmin, cz);
+    bool inc = dzmin < P.reach_z + 4 * m;
+    if (inc && P.mode == MIA_RMU) inc = dxy2 + dzmin * dzmin < rr * rr;
+    return inc;
+}
+
 template <class F>
 __device__ __forceinline__ void slab_runs(const PlanSample &P, double zlo, double zhi, double dxy2, double rr,
                                           F &&close) {
     const Grid &g = P.g;
-    const double m = g.margin;
-    const double rz = P.reach_z + 4 * m;
     int run_k0 = -1, run_cls = 0, run_jz = 0;
     for (int k = 0; k <= g.nz; ++k) {
         bool inc = false;
         int cz = 0;
-        if (k < g.nz) {
-            double dzmin;
-            axis_class(zlo - (k + 1) * g.hz - 2 * m, zhi - k * g.hz + 2 * m, g.L, g.half, P.periodic, dzmin, cz);
-            inc = dzmin < rz;
-            if (inc && P.mode == MIA_RMU) inc = dxy2 + dzmin * dzmin < rr * rr;
-        }
+        if (k < g.nz) inc = slab_in(P, zlo, zhi, k, dxy2, rr, cz);
         const int jz = k / g.nzj;
         if (run_k0 >= 0 && (!inc || cz != run_cls || jz != run_jz)) {
             close(run_k0, k, run_cls);
@@ -306,10 +313,19 @@ __device__ __forceinline__ void plan_ranges(const PlanSample &P, const Block &B,
         range(pb, pe, region_of(g, jx, jy, k0), cls);
     };
     if (runs && runs[0] >= 0) {
-        // (r_p, pi): the block's slab runs, the same for every column
+        // the block's slab runs: (r_p, pi) the same for every column; (r, mu)
+        // those of the block's own column, cut at both ends to this column's
+        // reach (the slabs a column reaches within one run are contiguous,
+        // the distance growing away from the block on either side)
         for (int r = 0; r < runs[0]; ++r) {
             const int v = runs[1 + r];
-            run(v & 0x7ff, (v >> 11) & 0x7ff, (v >> 22) & 3);
+            int k0 = v & 0x7ff, k1 = (v >> 11) & 0x7ff, cz;
+            if (P.mode == MIA_RMU) {
+                while (k0 < k1 && !slab_in(P, zlo, zhi, k0, dxy2, rr, cz)) ++k0;
+                while (k1 > k0 && !slab_in(P, zlo, zhi, k1 - 1, dxy2, rr, cz)) --k1;
+                if (k0 >= k1) continue;
+            }
+            run(k0, k1, (v >> 22) & 3);
         }
     } else {
         slab_runs(P, zlo, zhi, dxy2, rr, run);
@@ -319,7 +335,7 @@ __device__ __forceinline__ void plan_ranges(const PlanSample &P, const Block &B,
 }
 
 constexpr int kPlanThreads = 256;
-constexpr int kBlockRuns = 15;          // slab runs of a block kept for its columns ((r_p, pi))
+constexpr int kBlockRuns = 15;          // slab runs of a block kept for its columns
 constexpr int kPlanGridPerCu = 16;      // workgroups per CU of the planner's persistent grid
 constexpr int64_t kItemOverhead = 256;  // per-item cost, in pair evaluations
 constexpr int kOpenItems = 4;           // items a plan thread fills at once (one per key and shift)
@@ -735,13 +751,14 @@ __global__ void k_blocks(const PlanSample *ps, const int64_t *cell0, int S, int 
         }
         if (bruns) {
             int32_t *br = bruns + (kBlockRuns + 1) * (int64_t)(o + b);
+            // (r, mu): the runs of a column at zero transverse distance, the
+            // widest reach of any column
             int nr = 0;
-            if (P.mode == MIA_RPPI)
-                slab_runs(P, lo[2], hi[2], 0.0, 0.0, [&](int k0, int k1, int cls) {
-                    if (nr < kBlockRuns) br[1 + nr] = k0 | (k1 << 11) | (cls << 22);
-                    ++nr;
-                });
-            br[0] = P.mode == MIA_RPPI && nr <= kBlockRuns ? nr : -1;
+            slab_runs(P, lo[2], hi[2], 0.0, P.reach_r + 4 * g.margin, [&](int k0, int k1, int cls) {
+                if (nr < kBlockRuns) br[1 + nr] = k0 | (k1 << 11) | (cls << 22);
+                ++nr;
+            });
+            br[0] = nr <= kBlockRuns ? nr : -1;
         }
     }
 }
