@@ -1076,10 +1076,12 @@ __device__ void flush_slot(const PairArgs &P, char *hist, int64_t slot, int64_t 
 }
 
 
-__device__ __forceinline__ double r2_column_order(double a2, double b2, double l2, int los) {
-    if (los == 2) return (a2 + b2) + l2;
-    if (los == 1) return (a2 + l2) + b2;
-    return (l2 + a2) + b2;
+// r^2 summed in the reference's column order x, y, z: (a^2 + b^2) + l^2 with
+// the line of sight along z (LZ), else (a^2 + l^2) + b^2 (line of sight x:
+// (l^2 + a^2) + b^2, the same value since FP64 addition commutes).
+template <bool LZ>
+__device__ __forceinline__ double r2_column_order(double s, double a2, double b2, double l2) {
+    return LZ ? s + l2 : (a2 + l2) + b2;
 }
 
 // #{k in 1..n-1 : E[k] <= v}, the exact bin of v in [E[0], E[n]).
@@ -1155,7 +1157,7 @@ __device__ __forceinline__ void hist_add(const PairArgs &P, char *hist, int b, u
     if (VAR) atomicAdd(&((double *)(hist + P.off_var))[b], vr);
 }
 
-template <int MODE, int SH, bool WEIGHTED, bool VAR>
+template <int MODE, int SH, bool WEIGHTED, bool VAR, bool LZ>
 __device__ __forceinline__ void one_pair(const PairArgs &P, const SampleView &S, bool valid,
                                          double pa, double pb, double pl, double pw, double qa,
                                          double qb, double ql, double qax, double qay, double qew,
@@ -1182,7 +1184,7 @@ __device__ __forceinline__ void one_pair(const PairArgs &P, const SampleView &S,
         rv = s;
         in = valid & (s >= P.T0) & (s < P.Tn) & (dl >= P.X0) & (dl < P.Xn);
     } else {
-        rv = r2_column_order(a2, b2, dl * dl, S.los);
+        rv = r2_column_order<LZ>(s, a2, b2, dl * dl);
         in = valid & (s > 0.0) & (rv >= P.T0) & (rv < P.Tn);
     }
     if (in) {
@@ -1235,9 +1237,10 @@ __device__ __forceinline__ void one_pair(const PairArgs &P, const SampleView &S,
         double c = fma(da, qay, -(db * qax));
         double r0 = __builtin_amdgcn_rcp(s);
         double inv = fma(r0, fma(-s, r0, 1.0), r0);
-        double u = q2ew * t;
-        double ep = fma(u * t, inv, -qew);
-        double ex = (u * fabs(c)) * inv;
+        // 2 e w t / r_p^2: e+ = that * t - e w, ex = that * |c|
+        double tw = (q2ew * t) * inv;
+        double ep = fma(tw, t, -qew);
+        double ex = tw * fabs(c);
         if (fabs(c) < fabs(t) * kSinExact) {
             // nearly parallel: the reference's d may round outside [-1, 1],
             // where arccos gives NaN and the pair adds no e+ (measure_w_box_jk.py:170-175)
@@ -1271,7 +1274,7 @@ __device__ __forceinline__ void one_pair(const PairArgs &P, const SampleView &S,
     }
 }
 
-template <int MODE, int SH, bool WEIGHTED, bool VAR>
+template <int MODE, int SH, bool WEIGHTED, bool VAR, bool LZ>
 __device__ __forceinline__ void run_item(const PairArgs &P, const SampleView &S, int p0, int np, int nr,
                                          int shifts, ShapePtr shp, int ns, char *hist,
                                          const double *T, const double *X, int lane,
@@ -1347,12 +1350,12 @@ __device__ __forceinline__ void run_item(const PairArgs &P, const SampleView &S,
             ShapePtr r1 = shp + 8 * (j + 1);
             const double b0 = r1[0], b1 = r1[1], b2 = r1[2], b3 = r1[3], b4 = r1[4], b5 = r1[5],
                          b6 = r1[6], b7 = r1[7];
-            one_pair<MODE, SH, WEIGHTED, VAR>(P, S, v0, pa0, pb0, pl0, pw0, qa, qb, ql, qax, qay, qew, qw,
+            one_pair<MODE, SH, WEIGHTED, VAR, LZ>(P, S, v0, pa0, pb0, pl0, pw0, qa, qb, ql, qax, qay, qew, qw,
                                          q2ew, sha, shb, shl, hist, T, X, nre, pd, vlgc, vxc);
             ShapePtr r2 = shp + 8 * (j + 2 < ns ? j + 2 : j + 1);
             const double c0 = r2[0], c1 = r2[1], c2 = r2[2], c3 = r2[3], c4 = r2[4], c5 = r2[5],
                          c6 = r2[6], c7 = r2[7];
-            one_pair<MODE, SH, WEIGHTED, VAR>(P, S, v0, pa0, pb0, pl0, pw0, b0, b1, b2, b3, b4, b5, b6,
+            one_pair<MODE, SH, WEIGHTED, VAR, LZ>(P, S, v0, pa0, pb0, pl0, pw0, b0, b1, b2, b3, b4, b5, b6,
                                          b7, sha, shb, shl, hist, T, X, nre, pd, vlgc, vxc);
             qa = c0;
             qb = c1;
@@ -1364,10 +1367,26 @@ __device__ __forceinline__ void run_item(const PairArgs &P, const SampleView &S,
             q2ew = c7;
         }
         if (j < ns)
-            one_pair<MODE, SH, WEIGHTED, VAR>(P, S, v0, pa0, pb0, pl0, pw0, qa, qb, ql, qax, qay, qew, qw,
+            one_pair<MODE, SH, WEIGHTED, VAR, LZ>(P, S, v0, pa0, pb0, pl0, pw0, qa, qb, ql, qax, qay, qew, qw,
                                          q2ew, sha, shb, shl, hist, T, X, nre, pd, vlgc, vxc);
     }
     if (pd.b >= 0) hist_add<WEIGHTED, VAR>(P, hist, pd.b, pd.c, pd.sp, pd.sx, pd.dd, pd.vr);
+}
+
+// An item's pair loop for its periodic-shift variant.
+template <int MODE, bool WEIGHTED, bool VAR, bool LZ>
+__device__ __forceinline__ void run_shifted(const PairArgs &P, const SampleView &S, int p0, int np, int nr,
+                                            int shifts, ShapePtr shp, int ns, char *hist, const double *T,
+                                            const double *X, int lane, uint32_t &nre, uint64_t &iterations) {
+    if (shifts & 64)
+        run_item<MODE, SH_WRAP, WEIGHTED, VAR, LZ>(P, S, p0, np, nr, shifts, shp, ns, hist, T, X, lane, nre,
+                                                   iterations);
+    else if (shifts & 63)
+        run_item<MODE, SH_CONST, WEIGHTED, VAR, LZ>(P, S, p0, np, nr, shifts, shp, ns, hist, T, X, lane, nre,
+                                                    iterations);
+    else
+        run_item<MODE, SH_NONE, WEIGHTED, VAR, LZ>(P, S, p0, np, nr, shifts, shp, ns, hist, T, X, lane, nre,
+                                                   iterations);
 }
 
 template <int MODE, bool WEIGHTED, bool VAR>
@@ -1432,14 +1451,13 @@ __global__ __launch_bounds__(kThreads) void k_pairs(PairArgs P) {
                 cur_rp = rp;
             }
             ShapePtr shp = (ShapePtr)(S.srec + 8 * (int64_t)s0);
-            if (shifts & 64)
-                run_item<MODE, SH_WRAP, WEIGHTED, VAR>(P, S, p0, np, nr, shifts, shp, ns, hist, T, X, lane, nre,
-                                                       iterations);
-            else if (shifts & 63)
-                run_item<MODE, SH_CONST, WEIGHTED, VAR>(P, S, p0, np, nr, shifts, shp, ns, hist, T, X, lane, nre,
+            // (r, mu) sums r^2 in an order fixed by the line of sight: one
+            // loop per order, so the pair loop holds no branch on it
+            if (MODE == MIA_RMU && S.los != 2)
+                run_shifted<MODE, WEIGHTED, VAR, false>(P, S, p0, np, nr, shifts, shp, ns, hist, T, X, lane, nre,
                                                         iterations);
             else
-                run_item<MODE, SH_NONE, WEIGHTED, VAR>(P, S, p0, np, nr, shifts, shp, ns, hist, T, X, lane, nre,
+                run_shifted<MODE, WEIGHTED, VAR, true>(P, S, p0, np, nr, shifts, shp, ns, hist, T, X, lane, nre,
                                                        iterations);
         }
     } while (sch.next(P.queue, lane, it0, it1));
