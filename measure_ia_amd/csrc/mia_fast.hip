@@ -1346,13 +1346,17 @@ __device__ __forceinline__ void run_item(const PairArgs &P, const SampleView &S,
         double qa = shp[0], qb = shp[1], ql = shp[2], qax = shp[3], qay = shp[4], qew = shp[5],
                qw = shp[6], q2ew = shp[7];
         int j = 0;
-        for (; j + 1 < ns; j += 2) {
-            ShapePtr r1 = shp + 8 * (j + 1);
+        // byte offsets of the records: 32-bit scalars added to the block's
+        // base address by the scalar loads themselves
+        const uint32_t olast = (uint32_t)(ns - 1) * 64u;
+        uint32_t o = 0;
+        for (; j + 1 < ns; j += 2, o += 128u) {
+            ShapePtr r1 = (ShapePtr)((const MIA_CONST char *)shp + (o + 64u));
             const double b0 = r1[0], b1 = r1[1], b2 = r1[2], b3 = r1[3], b4 = r1[4], b5 = r1[5],
                          b6 = r1[6], b7 = r1[7];
             one_pair<MODE, SH, WEIGHTED, VAR, LZ>(P, S, v0, pa0, pb0, pl0, pw0, qa, qb, ql, qax, qay, qew, qw,
                                          q2ew, sha, shb, shl, hist, T, X, nre, pd, vlgc, vxc);
-            ShapePtr r2 = shp + 8 * (j + 2 < ns ? j + 2 : j + 1);
+            ShapePtr r2 = (ShapePtr)((const MIA_CONST char *)shp + (o + 128u < olast ? o + 128u : olast));
             const double c0 = r2[0], c1 = r2[1], c2 = r2[2], c3 = r2[3], c4 = r2[4], c5 = r2[5],
                          c6 = r2[6], c7 = r2[7];
             one_pair<MODE, SH, WEIGHTED, VAR, LZ>(P, S, v0, pa0, pb0, pl0, pw0, b0, b1, b2, b3, b4, b5, b6,
