@@ -941,13 +941,18 @@ __global__ __launch_bounds__(kLcThreads) void k_lc_pairs(LcPairArgs P) {
                 CPtrD pr = (CPtrD)(P.prec + 8 * (int64_t)p0);
                 double a0 = pr[0], a1 = pr[1], a2 = pr[2], a3 = pr[3], a4 = pr[4], a5 = pr[5], a6 = pr[6];
                 int k = 0;
-                for (; k + 1 < np; k += 2) {
-                    CPtrD r1 = pr + 8 * (k + 1);
+                // byte offsets of the records: 32-bit scalars that the scalar
+                // loads add to the block's base address themselves
+                typedef const __attribute__((address_space(4))) char *CPtrC;
+                const uint32_t olast = (uint32_t)(np - 1) * 64u;
+                uint32_t ob = 0;
+                for (; k + 1 < np; k += 2, ob += 128u) {
+                    CPtrD r1 = (CPtrD)((CPtrC)pr + (ob + 64u));
                     const double b0 = r1[0], b1 = r1[1], b2 = r1[2], b3 = r1[3], b4 = r1[4], b5 = r1[5],
                                  b6 = r1[6];
                     lc_pair<MODE, SHAPES>(P, valid, qra, qdec, qchi, qc2, qs2, qw, a0, a1, a2, a3, a4, a5,
                                           a6, hist, T, X, nre, pd, vlgc, vxc);
-                    CPtrD r2 = pr + 8 * (k + 2 < np ? k + 2 : k + 1);
+                    CPtrD r2 = (CPtrD)((CPtrC)pr + (ob + 128u < olast ? ob + 128u : olast));
                     const double c0 = r2[0], c1 = r2[1], c2 = r2[2], c3 = r2[3], c4 = r2[4], c5 = r2[5],
                                  c6 = r2[6];
                     lc_pair<MODE, SHAPES>(P, valid, qra, qdec, qchi, qc2, qs2, qw, b0, b1, b2, b3, b4, b5,
