@@ -104,7 +104,11 @@ inline double dkey_inv(uint64_t k) {
 // Line-of-sight order inside a segment: chi quantised to qmax + 1 steps over
 // the sample's range, a non-decreasing function of chi.  One radix sort by
 // (segment << bits | quantised chi) orders a sample; searches compare the
-// same quantised values, so the order they rely on is exact.
+// same quantised values, so the order they rely on is exact.  The sort takes
+// 8 key bits per pass, so chi gets at least 12 bits and then what rounds the
+// key up to a whole pass (at most 24): a coarser quantum only lets a range
+// admit shapes up to one quantum outside its window, which the exact pi test
+// rejects.
 struct ChiQ {
     double lo, scale;
     uint64_t qmax;
@@ -119,7 +123,8 @@ __host__ __device__ __forceinline__ uint64_t chi_q(const ChiQ &q, double x) {
 
 inline ChiQ make_chiq(double lo, double hi, int seg_bits) {
     ChiQ q;
-    q.bits = std::max(1, std::min(24, 64 - seg_bits));
+    const int whole = (seg_bits + 12 + 7) / 8 * 8;
+    q.bits = std::max(1, std::min(std::min(24, whole - seg_bits), 64 - seg_bits));
     q.qmax = ((uint64_t)1 << q.bits) - 1;
     q.lo = lo;
     q.scale = (double)q.qmax / std::fmax(hi - lo, 1e-300);
