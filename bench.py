@@ -339,7 +339,7 @@ def main():
                          "WRITE_SIZE scale factors) and exit")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); the counter "
+        print(f"bench.py: the counter runs on one GPU only (asked for --gpus {a.gpus}); it "
               f"would shard by sample or shape-cell slab with one histogram all-reduce, which is not "
               f"built.", file=sys.stderr)
         sys.exit(2)
